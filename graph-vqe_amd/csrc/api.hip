@@ -1,0 +1,337 @@
+// C ABI of the gvmps library (declared in include/gvmps.h).
+//
+// Problems are host objects (plan.cpp).  A batch uploads the problems it
+// uses, lays out the per-instance arenas in HBM and launches the kernels on
+// the caller's stream.  Errors become a nonzero return plus a thread-local
+// message; nothing here exits the process.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+
+using namespace gv;
+
+
+namespace {
+thread_local std::string g_error;
+}
+
+namespace gv {
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+static T *upload(std::vector<void *> &owned, const T *host, size_t count) {
+  const size_t n = count;
+  if (count == 0) count = 1;  // never a null table
+  void *d = nullptr;
+  hip_check(hipMalloc(&d, sizeof(T) * count), "hipMalloc");
+  owned.push_back(d);
+  if (host && n > 0) hip_check(hipMemcpy(d, host, sizeof(T) * n, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  else hip_check(hipMemset(d, 0, sizeof(T) * count), "hipMemset");
+  return static_cast<T *>(d);
+}
+
+Batch::~Batch() {
+  if (device >= 0) hipSetDevice(device);
+  for (hipEvent_t &e : ev)
+    if (e) hipEventDestroy(e);
+  for (void *p : owned) hipFree(p);
+}
+
+// Phase boundaries: event i is recorded before phase i; the phase lengths
+// are read after the stream synchronises (collect).
+void Batch::mark(int i, hipStream_t s) {
+  if (!timing) return;
+  if (!ev[i]) hip_check(hipEventCreate(&ev[i]), "hipEventCreate");
+  hip_check(hipEventRecord(ev[i], s), "hipEventRecord");
+  ev_used = std::max(ev_used, i + 1);
+}
+
+void Batch::collect() {
+  if (!timing || ev_used < 2) return;
+  for (int i = 0; i + 1 < ev_used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) phase_ms[i] += ms;
+  }
+  ++phase_calls;
+  ev_used = 0;
+}
+
+Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_problems, int device) {
+  if (n_problems <= 0) throw std::runtime_error("empty batch");
+  int ndev = 0;
+  hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+  if (device < 0 || device >= ndev) throw std::runtime_error("invalid HIP device");
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  auto B = std::make_unique<Batch>();
+  B->device = device;
+  // problem tables
+  std::vector<ProbDev> pd(n_problems);
+  for (int j = 0; j < n_problems; ++j) {
+    const Problem &P = *problems[j]->p;
+    B->probs.push_back(problems[j]->p);
+    ProbDev &d = pd[j];
+    d.n = P.n;
+    d.nparams = P.nparams;
+    d.nprog = (int)P.prog.size();
+    d.max_bond = P.max_bond;
+    d.abs_cut = P.abs_cut;
+    d.rel_cut = P.rel_cut;
+    d.nterms = P.nterms;
+    d.nspill = P.nspill;
+    d.prog = upload(B->owned, P.prog.data(), P.prog.size());
+    d.gates = upload(B->owned, P.gates.data(), P.gates.size());
+    d.fixes = upload(B->owned, P.fixes.data(), P.fixes.size());
+    d.chains = upload(B->owned, P.chains.data(), P.chains.size());
+    d.steps = upload(B->owned, P.steps.data(), P.steps.size());
+    d.children = upload(B->owned, P.children.data(), P.children.size());
+    d.term_list = upload(B->owned, P.term_list.data(), P.term_list.size());
+    d.coeffs = upload(B->owned, P.coeffs.data(), P.coeffs.size());
+    std::vector<int> act(P.nterms, 0);
+    for (int k : P.active) act[k] = 1;
+    d.active = upload(B->owned, act.data(), act.size());
+  }
+  B->d_probs = upload(B->owned, pd.data(), pd.size());
+  // instances
+  long long site = 0, env = 0, spill = 0, term = 0, param = 0;
+  for (int j = 0; j < n_problems; ++j) {
+    const Problem &P = *problems[j]->p;
+    if (counts[j] < 0) throw std::runtime_error("negative instance count");
+    for (int c = 0; c < counts[j]; ++c) {
+      InstDev in{};
+      in.prob = j;
+      in.n = P.n;
+      in.site_off = site;
+      in.env_off = env;
+      in.spill_off = spill;
+      in.term_off = term;
+      in.param_off = param;
+      in.out = (int)B->inst.size();
+      B->inst.push_back(in);
+      site += (long long)P.n * GV_SLOT;
+      env += 2LL * (P.n + 1) * GV_ENV;
+      spill += (long long)P.nspill * GV_ENV;
+      term += P.nterms;
+      param += P.nparams;
+      B->max_gen = std::max(B->max_gen, P.ngen);
+    }
+  }
+  if (B->inst.empty()) throw std::runtime_error("empty batch");
+  B->n_terms_total = term;
+  B->n_params_total = param;
+  B->d_inst = upload(B->owned, B->inst.data(), B->inst.size());
+  B->arena = upload<double2>(B->owned, nullptr, site);
+  B->envs = upload<double2>(B->owned, nullptr, env);
+  B->spills = upload<double2>(B->owned, nullptr, spill);
+  B->terms = upload<double2>(B->owned, nullptr, term);
+  B->status = upload<int>(B->owned, nullptr, B->inst.size());
+  B->bonds = upload<int>(B->owned, nullptr, B->inst.size() * (GV_NMAX + 1));
+  B->energies = upload<double>(B->owned, nullptr, B->inst.size());
+  // chain work items per generation, instance-major
+  for (int g = 0; g < B->max_gen; ++g) {
+    std::vector<int2> items;
+    for (size_t i = 0; i < B->inst.size(); ++i) {
+      const Problem &P = *B->probs[B->inst[i].prob];
+      if (g >= P.ngen) continue;
+      for (int c = P.gen_begin[g]; c < P.gen_begin[g + 1]; ++c) items.push_back(make_int2((int)i, c));
+    }
+    B->items.push_back(upload(B->owned, items.data(), items.size()));
+    B->n_items.push_back((int)items.size());
+  }
+  return B.release();
+}
+
+void check_status(Batch &B, hipStream_t stream) {
+  std::vector<int> st(B.inst.size());
+  hip_check(hipMemcpyAsync(st.data(), B.status, sizeof(int) * st.size(), hipMemcpyDeviceToHost, stream),
+            "hipMemcpyAsync status");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  B.collect();
+  for (size_t i = 0; i < st.size(); ++i) {
+    if (st[i] == ST_GUARD) {
+      const Problem &P = *B.probs[B.inst[i].prob];
+      throw std::runtime_error("CUDAQ_MPS_MAX_BOND=" + std::to_string(P.max_bond) +
+                               " is insufficient for zero-cutoff simulation: resolved singular values would be "
+                               "discarded");  // dist_svd.cuh:277-279
+    }
+    if (st[i] != ST_OK)
+      throw std::runtime_error("Direct MPS SVD failed or produced nonfinite singular values; devInfo=0");
+  }
+}
+
+void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
+  const int ninst = (int)B.inst.size();
+  B.mark(0, stream);
+  hip_check(kl_prep(ninst, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status), "k_prep launch");
+  B.mark(1, stream);
+}
+
+void launch_energy_tail(Batch &B, hipStream_t stream) {
+  const int ninst = (int)B.inst.size();
+  hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
+  B.mark(2, stream);
+  for (int g = 0; g < B.max_gen; ++g) {
+    const int ni = B.n_items[g];
+    if (ni == 0) continue;
+    hip_check(kl_chains(ni, stream, B.items[g], B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.terms),
+              "k_chains launch");
+  }
+  B.mark(3, stream);
+  hip_check(kl_reduce(ninst, stream, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
+  B.mark(4, stream);
+}
+
+}  // namespace gv
+
+// ------------------------------------------------------------------ C ABI
+#define GV_TRY(body)                        \
+  try {                                     \
+    g_error.clear();                        \
+    body;                                   \
+    return 0;                               \
+  } catch (const std::exception &ex) {      \
+    g_error = ex.what();                    \
+    return 1;                               \
+  } catch (...) {                           \
+    g_error = "unknown error";              \
+    return 1;                               \
+  }
+
+extern "C" {
+
+const char *gvmps_last_error(void) { return g_error.c_str(); }
+
+int gvmps_problem_create(gvmps_problem **out, int n_qubits, const gvmps_op *ops, int n_ops, int n_params,
+                         int max_bond, double abs_cut, double rel_cut, const char *words, const double *coeffs,
+                         int n_terms) {
+  GV_TRY({
+    if (!out) throw std::runtime_error("null output pointer");
+    auto p = std::make_unique<gvmps_problem>();
+    p->p = compile_problem(n_qubits, ops, n_ops, n_params, max_bond, abs_cut, rel_cut, words, coeffs, n_terms)
+               .release();
+    *out = p.release();
+  })
+}
+
+int gvmps_problem_destroy(gvmps_problem *p) {
+  GV_TRY({
+    if (p) {
+      delete p->p;
+      delete p;
+    }
+  })
+}
+
+int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
+  GV_TRY({
+    if (!p || !info) throw std::runtime_error("null pointer");
+    const Problem &P = *p->p;
+    std::memset(info, 0, sizeof(*info));
+    info->n_qubits = P.n;
+    info->n_params = P.nparams;
+    info->n_terms = P.nterms;
+    info->n_active_terms = (int)P.active.size();
+    info->n_prog_ops = (int)P.prog.size();
+    info->n_two_site_gates = P.n_g2;
+    info->max_alloc_bond = P.max_alloc_bond;
+    info->n_chains = (int)P.chains.size();
+    info->n_steps = (int)P.steps.size();
+    info->n_spill = P.nspill;
+    info->n_generations = P.ngen;
+    info->n_child_nodes = P.n_nodes;
+    info->flops_energy_executed = P.flops_exec;
+    info->flops_energy_nodes = P.flops_nodes;
+    info->flops_svd_nominal = P.flops_svd;
+  })
+}
+
+int gvmps_batch_create(gvmps_batch **out, gvmps_problem *const *problems, const int *counts, int n_problems,
+                       int device) {
+  GV_TRY({
+    if (!out || !problems || !counts) throw std::runtime_error("null pointer");
+    *out = reinterpret_cast<gvmps_batch *>(make_batch(problems, counts, n_problems, device));
+  })
+}
+
+int gvmps_batch_destroy(gvmps_batch *b) { GV_TRY({ delete reinterpret_cast<Batch *>(b); }) }
+
+int gvmps_batch_size(const gvmps_batch *b) {
+  return b ? (int)reinterpret_cast<const Batch *>(b)->inst.size() : 0;
+}
+
+int gvmps_batch_prepare(gvmps_batch *b, const double *params_dev, void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    launch_prepare(B, params_dev, (hipStream_t)stream);
+    check_status(B, (hipStream_t)stream);
+  })
+}
+
+int gvmps_batch_energy(gvmps_batch *b, const double *params_dev, double *energies_dev, double *terms_dev,
+                       void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hipStream_t s = (hipStream_t)stream;
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    launch_prepare(B, params_dev, s);
+    launch_energy_tail(B, s);
+    if (energies_dev)
+      hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * B.inst.size(),
+                               hipMemcpyDeviceToDevice, s),
+                "copy energies");
+    if (terms_dev)
+      hip_check(hipMemcpyAsync(terms_dev, B.terms, sizeof(double2) * B.n_terms_total, hipMemcpyDeviceToDevice, s),
+                "copy terms");
+    check_status(B, s);
+  })
+}
+
+int gvmps_batch_set_timing(gvmps_batch *b, int enable) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    B.timing = enable != 0;
+    B.ev_used = 0;
+    B.phase_calls = 0;
+    for (double &v : B.phase_ms) v = 0.0;
+  })
+}
+
+int gvmps_batch_get_timing(gvmps_batch *b, double *phase_ms, long long *calls) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    for (int i = 0; i < 8; ++i) phase_ms[i] = B.phase_ms[i];
+    if (calls) *calls = B.phase_calls;
+  })
+}
+
+int gvmps_batch_read_sites(gvmps_batch *b, int inst, double *sites_host, int *bonds_host) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (inst < 0 || inst >= (int)B.inst.size()) throw std::runtime_error("instance index out of range");
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const InstDev &in = B.inst[inst];
+    if (sites_host)
+      hip_check(hipMemcpy(sites_host, B.arena + in.site_off, sizeof(double2) * (size_t)in.n * GV_SLOT,
+                          hipMemcpyDeviceToHost),
+                "read sites");
+    if (bonds_host)
+      hip_check(hipMemcpy(bonds_host, B.bonds + (size_t)inst * (GV_NMAX + 1), sizeof(int) * (in.n + 1),
+                          hipMemcpyDeviceToHost),
+                "read bonds");
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// Host-side batch object shared by the C ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "plan.h"
+
+struct gvmps_problem {
+  gv::Problem *p = nullptr;
+};
+
+namespace gv {
+
+struct Batch {
+  int device = -1;
+  std::vector<const Problem *> probs;
+  std::vector<InstDev> inst;
+  std::vector<void *> owned;  // every device allocation of this batch
+  ProbDev *d_probs = nullptr;
+  InstDev *d_inst = nullptr;
+  double2 *arena = nullptr, *envs = nullptr, *spills = nullptr, *terms = nullptr;
+  int *status = nullptr, *bonds = nullptr;
+  double *energies = nullptr;
+  std::vector<int2 *> items;
+  std::vector<int> n_items;
+  int max_gen = 0;
+  long long n_terms_total = 0, n_params_total = 0;
+  // optional per-phase HIP-event timing (prep, envs, chains, reduce, sample, grad)
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  int ev_used = 0;
+  double phase_ms[8] = {};
+  long long phase_calls = 0;
+  void mark(int i, hipStream_t s);
+  void collect();
+  ~Batch();
+};
+
+void set_error(const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_problems, int device);
+void check_status(Batch &B, hipStream_t stream);
+void launch_prepare(Batch &B, const double *params, hipStream_t stream);
+void launch_energy_tail(Batch &B, hipStream_t stream);
+
+// kernel launchers (kernels_energy.hip)
+hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, const double *params,
+                   double2 *arena, int *bonds, int *status);
+hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
+hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                     const double2 *arena, const double2 *envs, double2 *spills, double2 *terms);
+hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
+                     double *energies);
+
+}  // namespace gv

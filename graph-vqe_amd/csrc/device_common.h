@@ -1,0 +1,129 @@
+// Device helpers shared by the gfx950 kernels: complex128 arithmetic and the
+// FP64 MFMA 16x16x4 fragment machinery used for environment transfers.
+//
+// MFMA operand layout (v_mfma_f64_16x16x4_f64, wave64):
+//   A (16x4):  lane l holds A[l % 16][l / 16]
+//   B (4x16):  lane l holds B[l / 16][l % 16]
+//   D (16x16): lane l, register v holds D[4*(l/16) + v][l % 16]
+// A 16x16x16 product is 4 k-steps; in k-step t lane group g = l/16 supplies
+// contraction index k = 4g + t.  With that k order the D registers of one
+// product are directly the A operand of the next (see Env below), so a
+// chain of transfers runs in registers with no lane shuffles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gvmps_internal.h"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
+  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// Environment E (16x16 complex) in "E layout": lane (a + 16 g), register t
+// holds E[a][4g + t].  Memory form is row-major E[a*16 + k].
+struct Env {
+  double re[4], im[4];
+};
+
+// Site tensor fragments: lane (j + 16 g) holds M_i[4g + t][j] for physical
+// index i, where M_i is A_i (left sweeps / chains) or A_i^T (right sweeps).
+struct Frag {
+  double re[2][4], im[2][4];
+};
+
+__device__ __forceinline__ void env_load(const double2 *p, int lane, Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+  const double2 *q = p + a * 16 + 4 * g;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    double2 v = q[t];
+    e.re[t] = v.x;
+    e.im[t] = v.y;
+  }
+}
+
+__device__ __forceinline__ void env_store(double2 *p, int lane, const Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+  double2 *q = p + a * 16 + 4 * g;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) q[t] = make_double2(e.re[t], e.im[t]);
+}
+
+__device__ __forceinline__ void env_unit(int lane, Env &e) {  // [1] in the (0,0) corner
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    e.re[t] = (lane == 0 && t == 0) ? 1.0 : 0.0;
+    e.im[t] = 0.0;
+  }
+}
+
+// Site slot layout: element (l, i, r) at (l*2 + i)*16 + r.
+__device__ __forceinline__ void frag_load(const double2 *slot, int lane, Frag &f, bool transposed) {
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int row = 4 * g + t;
+      const int idx = transposed ? ((j * 2 + i) * 16 + row) : ((row * 2 + i) * 16 + j);
+      double2 v = slot[idx];
+      f.re[i][t] = v.x;
+      f.im[i][t] = v.y;
+    }
+}
+
+// X_i = E * M_i  (result in D layout: lane (j+16g), reg v = X_i[4g+v][j]).
+__device__ __forceinline__ void prod_x(const Env &e, const Frag &f, int i, d4 &xr, d4 &xi) {
+  xr = (d4){0.0, 0.0, 0.0, 0.0};
+  xi = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    xr = mfma(e.re[t], f.re[i][t], xr);
+    xr = mfma(-e.im[t], f.im[i][t], xr);
+    xi = mfma(e.re[t], f.im[i][t], xi);
+    xi = mfma(e.im[t], f.re[i][t], xi);
+  }
+}
+
+// Y = M_j^H X, returned in E layout (computed as Y^T = X^T conj(M_j)).
+__device__ __forceinline__ void prod_y(const d4 &xr, const d4 &xi, const Frag &f, int j, d4 &yr, d4 &yi) {
+  yr = (d4){0.0, 0.0, 0.0, 0.0};
+  yi = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    yr = mfma(xr[t], f.re[j][t], yr);
+    yr = mfma(xi[t], f.im[j][t], yr);
+    yi = mfma(xi[t], f.re[j][t], yi);
+    yi = mfma(-xr[t], f.im[j][t], yi);
+  }
+}
+
+// Identity transfer T_I(E) = sum_i M_i^H E M_i.
+__device__ __forceinline__ void transfer_identity(const Env &e, const Frag &f, Env &out) {
+  d4 xr, xi, y0r, y0i, y1r, y1i;
+  prod_x(e, f, 0, xr, xi);
+  prod_y(xr, xi, f, 0, y0r, y0i);
+  prod_x(e, f, 1, xr, xi);
+  prod_y(xr, xi, f, 1, y1r, y1i);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    out.re[v] = y0r[v] + y1r[v];
+    out.im[v] = y0i[v] + y1i[v];
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}

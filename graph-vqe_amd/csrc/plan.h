@@ -1,0 +1,45 @@
+// Host planner interface (see plan.cpp).
+#pragma once
+#include <array>
+#include <complex>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gvmps.h"
+#include "gvmps_internal.h"
+
+namespace gv {
+
+using cd = std::complex<double>;
+
+struct Problem {
+  int n = 0, nparams = 0, max_bond = 0;
+  double abs_cut = 0, rel_cut = 0;
+  // circuit
+  std::vector<ProgOp> prog;
+  std::vector<double> gates;   // 32 doubles per two-site gate
+  std::vector<double> fixes;   // 8 doubles per fixed single-site gate
+  std::vector<int> bonds_pred; // n+1, zero-cutoff allocation
+  int max_alloc_bond = 1, n_g2 = 0;
+  double flops_svd = 0;
+  // Hamiltonian
+  int nterms = 0;
+  std::vector<double> coeffs;            // 2*nterms
+  std::vector<std::string> site_words;   // site order
+  std::vector<int> active;               // non-identity terms
+  // prefix forest, chain schedule
+  std::vector<ChainDev> chains;
+  std::vector<StepDev> steps;
+  std::vector<ChildDev> children;
+  std::vector<int> term_list;
+  std::vector<int> gen_begin;            // chains of generation g: [gen_begin[g], gen_begin[g+1])
+  int nspill = 0, ngen = 0, n_nodes = 0;
+  double flops_exec = 0, flops_nodes = 0;
+};
+
+std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int n_ops, int n_params,
+                                         int max_bond, double abs_cut, double rel_cut,
+                                         const char *words, const double *coeffs, int n_terms);
+
+}  // namespace gv

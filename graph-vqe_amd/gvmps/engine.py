@@ -1,0 +1,157 @@
+"""Host mirror of the batched MPS engine over the C ABI.
+
+`Problem` = one circuit tape + one Hamiltonian compiled by the native planner;
+`Batch` = instances of one or more problems evaluated by single launches.
+Device buffers are torch tensors on the current HIP device; the stream is
+torch's current stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+# reference defaults: CUDAQ_MPS_MAX_BOND = 64, cutoffs 1e-5
+# (test/backend/source/simulator_mps_dist.cu:363-366, class defaults)
+DEFAULT_MAX_BOND = 64
+DEFAULT_CUTOFF = 1e-5
+
+
+def encode_ops(ops):
+    """Tape -> gvmps_op array.  An op is (name, qubits, ref) where ref is None
+    (fixed gate), an int parameter index, a float constant angle, or a tuple
+    (param_index, scale, offset) for affine parameter expressions."""
+    arr = (_lib.Op * max(1, len(ops)))()
+    for k, (name, qubits, ref) in enumerate(ops):
+        name = name.lower()
+        if name not in _lib.GATE:
+            raise RuntimeError(f"tensornet-mps: unsupported gate '{name}'")
+        o = arr[k]
+        o.kind = _lib.GATE[name]
+        o.q0 = int(qubits[0])
+        o.q1 = int(qubits[1]) if len(qubits) > 1 else -1
+        o.param, o.scale, o.offset = -1, 0.0, 0.0
+        if name in ('rx', 'ry', 'rz'):
+            if isinstance(ref, (int, np.integer)):
+                o.param, o.scale = int(ref), 1.0
+            elif isinstance(ref, tuple):
+                o.param, o.scale, o.offset = int(ref[0]), float(ref[1]), float(ref[2])
+            else:
+                o.offset = float(ref)
+    return arr
+
+
+class Problem:
+    def __init__(self, n_qubits, ops, n_params, words, coeffs, max_bond=DEFAULT_MAX_BOND,
+                 abs_cut=DEFAULT_CUTOFF, rel_cut=DEFAULT_CUTOFF):
+        self.n = int(n_qubits)
+        self.n_params = int(n_params)
+        self.words = list(words)
+        c = np.asarray(coeffs, dtype=complex).reshape(-1)
+        if len(c) != len(self.words):
+            raise ValueError("words/coeffs length mismatch")
+        for w in self.words:
+            if len(w) != self.n:
+                raise ValueError("Invalid analytic Pauli width")
+        self.coeffs = c
+        cbuf = np.ascontiguousarray(np.stack([c.real, c.imag], axis=1).reshape(-1), dtype=np.float64)
+        wbuf = ''.join(self.words).encode()
+        opa = encode_ops(ops)
+        h = ctypes.c_void_p()
+        check(lib().gvmps_problem_create(ctypes.byref(h), self.n, opa, len(ops), self.n_params, int(max_bond),
+                                         float(abs_cut), float(rel_cut), wbuf,
+                                         cbuf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(self.words)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = _lib.ProblemInfo()
+        check(lib().gvmps_problem_get_info(self._h, ctypes.byref(inf)))
+        return {name: getattr(inf, name) for name, _ in inf._fields_}
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.gvmps_problem_destroy(h)
+            self._h = None
+
+
+def _stream_ptr(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Batch:
+    """counts[j] instances of problems[j]; parameters are concatenated in order."""
+
+    def __init__(self, problems, counts, device=None):
+        import torch
+        self.torch = torch
+        self.problems = list(problems)
+        self.counts = [int(c) for c in counts]
+        self.device = torch.cuda.current_device() if device is None else int(device)
+        harr = (ctypes.c_void_p * len(self.problems))(*[p.handle.value for p in self.problems])
+        carr = (ctypes.c_int * len(self.counts))(*self.counts)
+        h = ctypes.c_void_p()
+        check(lib().gvmps_batch_create(ctypes.byref(h), harr, carr, len(self.problems), self.device))
+        self._h = h
+        self.size = sum(self.counts)
+        self.n_params_total = sum(p.n_params * c for p, c in zip(self.problems, self.counts))
+        self.n_terms_total = sum(len(p.words) * c for p, c in zip(self.problems, self.counts))
+
+    def _params(self, params):
+        t = self.torch
+        x = t.as_tensor(params, dtype=t.float64, device=f'cuda:{self.device}').reshape(-1).contiguous()
+        if x.numel() != self.n_params_total:
+            raise ValueError(f"expected {self.n_params_total} parameters, got {x.numel()}")
+        return x
+
+    def energies(self, params, terms=False):
+        """Analytic energies (and optionally per-term <P_k> as complex)."""
+        t = self.torch
+        x = self._params(params)
+        out = t.empty(self.size, dtype=t.float64, device=x.device)
+        tv = t.empty(2 * self.n_terms_total, dtype=t.float64, device=x.device) if terms else None
+        check(lib().gvmps_batch_energy(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                       ctypes.c_void_p(tv.data_ptr()) if terms else None, _stream_ptr(t)))
+        if terms:
+            return out, t.view_as_complex(tv.view(-1, 2))
+        return out
+
+    def prepare(self, params):
+        t = self.torch
+        x = self._params(params)
+        check(lib().gvmps_batch_prepare(self._h, ctypes.c_void_p(x.data_ptr()), _stream_ptr(t)))
+
+    def set_timing(self, enable=True):
+        check(lib().gvmps_batch_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """Accumulated per-phase device milliseconds and the number of calls."""
+        ms = (ctypes.c_double * 8)()
+        calls = ctypes.c_longlong()
+        check(lib().gvmps_batch_get_timing(self._h, ms, ctypes.byref(calls)))
+        return list(ms), calls.value
+
+    def read_sites(self, inst):
+        """Prepared site tensors of instance `inst` as numpy (n, 16, 2, 16) + bonds."""
+        j, k = 0, inst
+        while k >= self.counts[j]:
+            k -= self.counts[j]
+            j += 1
+        n = self.problems[j].n
+        buf = np.zeros(n * 16 * 2 * 16 * 2, dtype=np.float64)
+        bonds = np.zeros(n + 1, dtype=np.int32)
+        check(lib().gvmps_batch_read_sites(self._h, int(inst), buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           bonds.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        z = buf.view(np.complex128).reshape(n, 16, 2, 16)
+        return z, bonds
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.gvmps_batch_destroy(h)
+            self._h = None

@@ -1,0 +1,110 @@
+/*
+ * gvmps -- MI355X (gfx950) MPS simulator backend: C ABI.
+ *
+ * Drop-in boundary for the hot path of the reference's custom MPS NVQIR
+ * plugin (test/backend/source/simulator_mps_dist.cu) and its reverse-gradient
+ * engine (test/tools/reverse_mps.py).  Plain pointers and sizes only; device
+ * pointers are HIP device memory, `stream` is a hipStream_t (NULL = default).
+ *
+ * Every entry point returns 0 on success and a nonzero status on failure; the
+ * message is then available from gvmps_last_error() (thread-local).  No entry
+ * point calls exit() (the reference's CUDA_CHECK does, dist_svd.cuh:22-27).
+ */
+#ifndef GVMPS_API_H_
+#define GVMPS_API_H_
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Gate kinds of a circuit tape (CUDA-Q builder ops recorded by
+ * test/tools/circuit_tape.py:56-66). */
+enum gvmps_gate {
+  GVMPS_RX = 0, GVMPS_RY = 1, GVMPS_RZ = 2,
+  GVMPS_H = 3, GVMPS_X = 4, GVMPS_Y = 5, GVMPS_Z = 6, GVMPS_S = 7, GVMPS_SDG = 8,
+  GVMPS_CX = 9, GVMPS_SWAP = 10
+};
+
+/* One tape entry, in CUDA-Q qubit indices.  Rotations take the angle
+ * scale * x[param] + offset (param = -1: the constant `offset`); this covers
+ * the affine parameter expressions of circuit_tape.Parameter. */
+typedef struct gvmps_op {
+  int32_t kind;
+  int32_t q0, q1;
+  int32_t param;
+  double scale, offset;
+} gvmps_op;
+
+typedef struct gvmps_problem gvmps_problem;
+typedef struct gvmps_batch gvmps_batch;
+
+typedef struct gvmps_problem_info {
+  int32_t n_qubits, n_params, n_terms, n_active_terms;
+  int32_t n_prog_ops, n_two_site_gates;
+  int32_t max_alloc_bond;          /* predicted allocated bond (zero-cutoff rule) */
+  int32_t n_chains, n_steps, n_spill, n_generations;
+  int32_t n_child_nodes;           /* distinct prefix nodes (spans only) */
+  double flops_energy_executed;    /* FP64 flops of one energy eval, as executed */
+  double flops_energy_nodes;       /* prefix-node algorithmic count, 16(Dl^2Dr+DlDr^2) per node */
+  double flops_svd_nominal;        /* Golub-Reinsch nominal count of all SVDs */
+} gvmps_problem_info;
+
+/* Last error message of the calling thread ("" if none). */
+const char *gvmps_last_error(void);
+
+/* Compile a circuit + Hamiltonian.  Replaces the plugin's per-gate
+ * applyGate (simulator_mps_dist.cu:1392-1406, routing :392-416, controlled
+ * gates :374-384, site = n-1-q :386-387), its engine configuration
+ * (CUDAQ_MPS_MAX_BOND / ABS_CUTOFF / RELATIVE_CUTOFF, :363-366) and the Pauli
+ * word construction of observe (buildPauliTermData :571-583).
+ *   words : n_terms * n_qubits chars 'I','X','Y','Z', word[q] acts on qubit q
+ *   coeffs: 2 * n_terms doubles (re, im)
+ * Host-only: needs no GPU. */
+int gvmps_problem_create(gvmps_problem **out, int n_qubits, const gvmps_op *ops, int n_ops,
+                         int n_params, int max_bond, double abs_cut, double rel_cut,
+                         const char *words, const double *coeffs, int n_terms);
+int gvmps_problem_destroy(gvmps_problem *p);
+int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info);
+
+/* A batch of MPS instances: counts[j] instances of problems[j], in order.
+ * Allocates the device workspace on `device`.  Replaces the per-thread
+ * ShardedMPS / workspace ownership (sharded_mps.cuh:309-353, 614-665): one
+ * batch holds every instance of a persistent launch. */
+int gvmps_batch_create(gvmps_batch **out, gvmps_problem *const *problems, const int *counts,
+                       int n_problems, int device);
+int gvmps_batch_destroy(gvmps_batch *b);
+int gvmps_batch_size(const gvmps_batch *b);
+
+/* Analytic energies (shots = -1 observe).  Replaces observe()'s analytic path
+ * (simulator_mps_dist.cu:1434-1485) over expectPauliShared
+ * (sharded_mps.cuh:614-687, analytic_batch.cuh:4-36), for every instance:
+ *   params_dev  : concatenated parameter vectors (instance order), doubles
+ *   energies_dev: one double per instance, Re sum_k c_k <P_k>
+ *   terms_dev   : optional, per instance n_terms complex <P_k> (re, im)  */
+int gvmps_batch_energy(gvmps_batch *b, const double *params_dev, double *energies_dev,
+                       double *terms_dev, void *stream);
+
+/* State preparation only (gates + SVDs).  Replaces the applyGate stream
+ * (sharded_mps.cuh:465-572 with dist_svd.cuh:194-283).  Returns nonzero with
+ * the reference's guard message when a zero-cutoff SVD would discard a
+ * resolved singular value (dist_svd.cuh:277-279). */
+int gvmps_batch_prepare(gvmps_batch *b, const double *params_dev, void *stream);
+
+/* Per-phase device timing with HIP events recorded on the launch stream.
+ * Phases: 0 state preparation, 1 identity environments, 2 prefix chains,
+ * 3 energy reduction.  enable != 0 resets the accumulators.  No reference
+ * counterpart (the reference times whole calls from Python,
+ * test/tools/analytic_execution.py:53-62); used by the benchmark. */
+int gvmps_batch_set_timing(gvmps_batch *b, int enable);
+int gvmps_batch_get_timing(gvmps_batch *b, double *phase_ms /* 8 */, long long *calls);
+
+/* Read back the prepared site tensors of instance `inst` (host buffer of
+ * n*16*2*16 complex) and its bond dimensions (n+1 ints).  Test/debug aid. */
+int gvmps_batch_read_sites(gvmps_batch *b, int inst, double *sites_host, int *bonds_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GVMPS_API_H_ */
