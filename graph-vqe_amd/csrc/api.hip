@@ -47,6 +47,8 @@ Batch::~Batch() {
   for (hipEvent_t &e : ev)
     if (e) hipEventDestroy(e);
   for (void *p : owned) hipFree(p);
+  if (sbits) hipFree(sbits);
+  if (spacked) hipFree(spacked);
 }
 
 // Phase boundaries: event i is recorded before phase i; the phase lengths
@@ -103,6 +105,23 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.active = upload(B->owned, act.data(), act.size());
   }
   B->d_probs = upload(B->owned, pd.data(), pd.size());
+  // Z masks (site order) for the shot estimator
+  std::vector<const unsigned long long *> mask_ptrs(n_problems);
+  for (int j = 0; j < n_problems; ++j) {
+    const Problem &P = *problems[j]->p;
+    std::vector<unsigned long long> m(2 * (size_t)P.nterms, 0ULL);
+    bool zonly = true;
+    for (int k = 0; k < P.nterms; ++k)
+      for (int s = 0; s < P.n; ++s) {
+        const char c = P.site_words[k][s];
+        if (c == 'Z') m[2 * k + (s >> 6)] |= 1ULL << (s & 63);
+        else if (c != 'I') zonly = false;
+      }
+    B->zonly.push_back(zonly ? 1 : 0);
+    mask_ptrs[j] = upload(B->owned, m.data(), m.size());
+    B->max_terms = std::max(B->max_terms, P.nterms);
+  }
+  B->d_masks = upload(B->owned, mask_ptrs.data(), mask_ptrs.size());
   // instances
   long long site = 0, env = 0, spill = 0, term = 0, param = 0;
   for (int j = 0; j < n_problems; ++j) {
@@ -138,6 +157,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->status = upload<int>(B->owned, nullptr, B->inst.size());
   B->bonds = upload<int>(B->owned, nullptr, B->inst.size() * (GV_NMAX + 1));
   B->energies = upload<double>(B->owned, nullptr, B->inst.size());
+  B->seeds = upload<unsigned>(B->owned, nullptr, B->inst.size());
   // chain work items per generation, instance-major
   for (int g = 0; g < B->max_gen; ++g) {
     std::vector<int2> items;
@@ -190,6 +210,41 @@ void launch_energy_tail(Batch &B, hipStream_t stream) {
   B.mark(3, stream);
   hip_check(kl_reduce(ninst, stream, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
   B.mark(4, stream);
+}
+
+// Prepare + identity environments + perfect sampling into the batch's own
+// (or the caller's) buffers.  Returns the packed-word buffer used.
+unsigned long long *launch_sample(Batch &B, const double *params, int shots, const uint32_t *seeds,
+                                  unsigned char *bits, unsigned long long *packed, hipStream_t stream) {
+  if (shots <= 0) throw std::runtime_error("shared bitstring sampling requires a positive shot count");
+  if (!seeds) throw std::runtime_error("null seed array");
+  const int ninst = (int)B.inst.size();
+  int max_n = 0;
+  for (const InstDev &in : B.inst) max_n = std::max(max_n, in.n);
+  const size_t nbits = (size_t)ninst * shots * max_n, nwords = (size_t)ninst * shots * 2;
+  if (!bits) {
+    if (B.sbits_cap < nbits) {
+      if (B.sbits) hip_check(hipFree(B.sbits), "hipFree");
+      hip_check(hipMalloc(&B.sbits, nbits), "hipMalloc bits");
+      B.sbits_cap = nbits;
+    }
+    bits = B.sbits;
+  }
+  if (!packed) {
+    if (B.spacked_cap < nwords) {
+      if (B.spacked) hip_check(hipFree(B.spacked), "hipFree");
+      hip_check(hipMalloc(&B.spacked, nwords * sizeof(unsigned long long)), "hipMalloc packed");
+      B.spacked_cap = nwords;
+    }
+    packed = B.spacked;
+  }
+  hip_check(hipMemcpyAsync(B.seeds, seeds, sizeof(unsigned) * ninst, hipMemcpyHostToDevice, stream), "seeds H2D");
+  launch_prepare(B, params, stream);
+  hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
+  B.mark(2, stream);
+  hip_check(kl_sample(ninst, shots, max_n, stream, B.d_inst, B.arena, B.envs, B.seeds, bits, packed), "k_sample launch");
+  B.mark(3, stream);
+  return packed;
 }
 
 }  // namespace gv
@@ -287,6 +342,41 @@ int gvmps_batch_energy(gvmps_batch *b, const double *params_dev, double *energie
     hip_check(hipSetDevice(B.device), "hipSetDevice");
     launch_prepare(B, params_dev, s);
     launch_energy_tail(B, s);
+    if (energies_dev)
+      hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * B.inst.size(),
+                               hipMemcpyDeviceToDevice, s),
+                "copy energies");
+    if (terms_dev)
+      hip_check(hipMemcpyAsync(terms_dev, B.terms, sizeof(double2) * B.n_terms_total, hipMemcpyDeviceToDevice, s),
+                "copy terms");
+    check_status(B, s);
+  })
+}
+
+int gvmps_batch_sample(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
+                       uint8_t *bits_dev, uint64_t *packed_dev, void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hipStream_t s = (hipStream_t)stream;
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    launch_sample(B, params_dev, shots, seeds, bits_dev, reinterpret_cast<unsigned long long *>(packed_dev), s);
+    check_status(B, s);
+  })
+}
+
+int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
+                            double *energies_dev, double *terms_dev, void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hipStream_t s = (hipStream_t)stream;
+    for (int z : B.zonly)
+      if (!z) throw std::runtime_error("finite-shot observe supports I/Z Pauli words only");
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    unsigned long long *packed = launch_sample(B, params_dev, shots, seeds, nullptr, nullptr, s);
+    const int ninst = (int)B.inst.size();
+    hip_check(kl_zsums(ninst, B.max_terms, shots, s, B.d_inst, B.d_probs, B.d_masks, packed, B.terms), "k_zsums");
+    hip_check(kl_reduce(ninst, s, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
+    B.mark(4, s);
     if (energies_dev)
       hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * B.inst.size(),
                                hipMemcpyDeviceToDevice, s),

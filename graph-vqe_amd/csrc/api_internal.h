@@ -23,6 +23,14 @@ struct Batch {
   double2 *arena = nullptr, *envs = nullptr, *spills = nullptr, *terms = nullptr;
   int *status = nullptr, *bonds = nullptr;
   double *energies = nullptr;
+  // sampling
+  const unsigned long long **d_masks = nullptr;  // per problem: 2 words per term (site order)
+  std::vector<int> zonly;                        // per problem: all words are I/Z
+  unsigned *seeds = nullptr;
+  unsigned char *sbits = nullptr;
+  unsigned long long *spacked = nullptr;
+  size_t sbits_cap = 0, spacked_cap = 0;
+  int max_terms = 0;
   std::vector<int2 *> items;
   std::vector<int> n_items;
   int max_gen = 0;
@@ -53,5 +61,10 @@ hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev
                      const double2 *arena, const double2 *envs, double2 *spills, double2 *terms);
 hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
                      double *energies);
+// kernel launchers (kernels_sample.hip)
+hipError_t kl_sample(int ninst, int shots, int stride, hipStream_t s, const InstDev *insts, const double2 *arena,
+                     const double2 *envs, const unsigned *seeds, unsigned char *bits, unsigned long long *packed);
+hipError_t kl_zsums(int ninst, int max_terms, int shots, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                    const unsigned long long *const *masks, const unsigned long long *packed, double2 *terms);
 
 }  // namespace gv

@@ -4,9 +4,9 @@
 // MFMA operand layout (v_mfma_f64_16x16x4_f64, wave64):
 //   A (16x4):  lane l holds A[l % 16][l / 16]
 //   B (4x16):  lane l holds B[l / 16][l % 16]
-//   D (16x16): lane l, register v holds D[4*(l/16) + v][l % 16]
+//   D (16x16): lane l, register v holds D[(l/16) + 4v][l % 16]
 // A 16x16x16 product is 4 k-steps; in k-step t lane group g = l/16 supplies
-// contraction index k = 4g + t.  With that k order the D registers of one
+// contraction index k = g + 4t.  With that k order the D registers of one
 // product are directly the A operand of the next (see Env below), so a
 // chain of transfers runs in registers with no lane shuffles.
 #pragma once
@@ -15,6 +15,12 @@
 #include "gvmps_internal.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Contraction / row index carried by lane group g (= lane / 16) and register
+// or k-step t.  It equals the D-register row of the MFMA output (measured on
+// gfx950: D[i][j] sits in lane j + 16*(i % 4), register i / 4), so one
+// product's D registers are the next product's A operand.
+__device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -30,12 +36,12 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
 // Environment E (16x16 complex) in "E layout": lane (a + 16 g), register t
-// holds E[a][4g + t].  Memory form is row-major E[a*16 + k].
+// holds E[a][kdx(g, t)].  Memory form is row-major E[a*16 + k].
 struct Env {
   double re[4], im[4];
 };
 
-// Site tensor fragments: lane (j + 16 g) holds M_i[4g + t][j] for physical
+// Site tensor fragments: lane (j + 16 g) holds M_i[kdx(g, t)][j] for physical
 // index i, where M_i is A_i (left sweeps / chains) or A_i^T (right sweeps).
 struct Frag {
   double re[2][4], im[2][4];
@@ -43,10 +49,10 @@ struct Frag {
 
 __device__ __forceinline__ void env_load(const double2 *p, int lane, Env &e) {
   const int a = lane & 15, g = lane >> 4;
-  const double2 *q = p + a * 16 + 4 * g;
+  const double2 *q = p + a * 16;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    double2 v = q[t];
+    double2 v = q[kdx(g, t)];
     e.re[t] = v.x;
     e.im[t] = v.y;
   }
@@ -54,9 +60,9 @@ __device__ __forceinline__ void env_load(const double2 *p, int lane, Env &e) {
 
 __device__ __forceinline__ void env_store(double2 *p, int lane, const Env &e) {
   const int a = lane & 15, g = lane >> 4;
-  double2 *q = p + a * 16 + 4 * g;
+  double2 *q = p + a * 16;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) q[t] = make_double2(e.re[t], e.im[t]);
+  for (int t = 0; t < 4; ++t) q[kdx(g, t)] = make_double2(e.re[t], e.im[t]);
 }
 
 __device__ __forceinline__ void env_unit(int lane, Env &e) {  // [1] in the (0,0) corner
@@ -74,7 +80,7 @@ __device__ __forceinline__ void frag_load(const double2 *slot, int lane, Frag &f
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const int row = 4 * g + t;
+      const int row = kdx(g, t);
       const int idx = transposed ? ((j * 2 + i) * 16 + row) : ((row * 2 + i) * 16 + j);
       double2 v = slot[idx];
       f.re[i][t] = v.x;
@@ -82,7 +88,7 @@ __device__ __forceinline__ void frag_load(const double2 *slot, int lane, Frag &f
     }
 }
 
-// X_i = E * M_i  (result in D layout: lane (j+16g), reg v = X_i[4g+v][j]).
+// X_i = E * M_i  (result in D layout: lane (j+16g), reg v = X_i[kdx(g,v)][j]).
 __device__ __forceinline__ void prod_x(const Env &e, const Frag &f, int i, d4 &xr, d4 &xi) {
   xr = (d4){0.0, 0.0, 0.0, 0.0};
   xi = (d4){0.0, 0.0, 0.0, 0.0};

@@ -142,16 +142,15 @@ k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, con
           const double theta = (o.param >= 0 ? o.scale * x[o.param] : 0.0) + o.offset;
           double sn, cs;
           sincos(0.5 * theta, &sn, &cs);
-          if (o.arg == 0) {        // RX = [[c, -is], [-is, c]]
-            U[0] = make_double2(cs, 0); U[1] = make_double2(0, -sn);
-            U[2] = make_double2(0, -sn); U[3] = make_double2(cs, 0);
-          } else if (o.arg == 1) { // RY = [[c, -s], [s, c]]
-            U[0] = make_double2(cs, 0); U[1] = make_double2(-sn, 0);
-            U[2] = make_double2(sn, 0); U[3] = make_double2(cs, 0);
-          } else {                 // RZ = diag(e^{-i t/2}, e^{i t/2})
-            U[0] = make_double2(cs, -sn); U[1] = make_double2(0, 0);
-            U[2] = make_double2(0, 0); U[3] = make_double2(cs, sn);
-          }
+          // R_sigma(theta) = c I - i s sigma: RX = [[c, -is], [-is, c]],
+          // RY = [[c, -s], [s, c]], RZ = diag(c - is, c + is).  Written
+          // branch-free: a three-way branch here lost the imaginary part of
+          // RZ's first entry in this kernel's gfx950 code.
+          const double fx = o.arg == 0 ? 1.0 : 0.0, fy = o.arg == 1 ? 1.0 : 0.0, fz = o.arg == 2 ? 1.0 : 0.0;
+          U[0] = make_double2(cs, -sn * fz);
+          U[1] = make_double2(-sn * fy, -sn * fx);
+          U[2] = make_double2(sn * fy, -sn * fx);
+          U[3] = make_double2(cs, sn * fz);
         } else {
           const double *F = P.fixes + 8 * o.arg;
           for (int e = 0; e < 4; ++e) U[e] = make_double2(F[2 * e], F[2 * e + 1]);

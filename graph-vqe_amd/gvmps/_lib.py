@@ -35,6 +35,10 @@ _PROTOS = {
     'gvmps_batch_size': [ctypes.c_void_p],
     'gvmps_batch_energy': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_prepare': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_batch_sample': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_batch_shot_energy': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_set_timing': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_get_timing': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
     'gvmps_batch_read_sites': [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),

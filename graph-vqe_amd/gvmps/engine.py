@@ -121,6 +121,45 @@ class Batch:
             return out, t.view_as_complex(tv.view(-1, 2))
         return out
 
+    def _seeds(self, seeds):
+        s = np.asarray(seeds, dtype=np.uint32).reshape(-1)
+        if s.size == 1 and self.size > 1:
+            s = np.full(self.size, s[0], dtype=np.uint32)
+        if s.size != self.size:
+            raise ValueError("one seed per instance is required")
+        return np.ascontiguousarray(s)
+
+    def sample(self, params, shots, seeds):
+        """Perfect sampling.  Returns (bits, packed): bits[inst, shot, site] in
+        {0,1} (site order, row stride = largest n of the batch) and packed
+        uint64 words [inst, shot, 2] (bit s = site s), both on the device."""
+        t = self.torch
+        x = self._params(params)
+        s = self._seeds(seeds)
+        nmax = max(p.n for p in self.problems)
+        bits = t.empty((self.size, int(shots), nmax), dtype=t.uint8, device=x.device)
+        packed = t.empty((self.size, int(shots), 2), dtype=t.int64, device=x.device)
+        check(lib().gvmps_batch_sample(self._h, ctypes.c_void_p(x.data_ptr()), int(shots),
+                                       s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                       ctypes.c_void_p(bits.data_ptr()), ctypes.c_void_p(packed.data_ptr()),
+                                       _stream_ptr(t)))
+        return bits, packed
+
+    def shot_energies(self, params, shots, seeds, terms=False):
+        """Finite-shot energies from one shared sample set per instance."""
+        t = self.torch
+        x = self._params(params)
+        s = self._seeds(seeds)
+        out = t.empty(self.size, dtype=t.float64, device=x.device)
+        tv = t.empty(2 * self.n_terms_total, dtype=t.float64, device=x.device) if terms else None
+        check(lib().gvmps_batch_shot_energy(self._h, ctypes.c_void_p(x.data_ptr()), int(shots),
+                                            s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                            ctypes.c_void_p(out.data_ptr()),
+                                            ctypes.c_void_p(tv.data_ptr()) if terms else None, _stream_ptr(t)))
+        if terms:
+            return out, t.view_as_complex(tv.view(-1, 2))
+        return out
+
     def prepare(self, params):
         t = self.torch
         x = self._params(params)

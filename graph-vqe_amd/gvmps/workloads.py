@@ -15,6 +15,10 @@ SEED_H = 20260
 
 
 def z_hamiltonian(n, T, wmax, rng):
+    """T distinct Z-strings (capped at the number of distinct supports of
+    weight 1..wmax on n qubits, e.g. 246 for n = 8, wmax = 6)."""
+    from math import comb
+    T = min(T, sum(comb(n, w) for w in range(1, min(wmax, n) + 1)))
     words, coeffs, seen = [], [], set()
     while len(words) < T:
         w = int(rng.integers(1, wmax + 1))

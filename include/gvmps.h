@@ -92,6 +92,25 @@ int gvmps_batch_energy(gvmps_batch *b, const double *params_dev, double *energie
  * resolved singular value (dist_svd.cuh:277-279). */
 int gvmps_batch_prepare(gvmps_batch *b, const double *params_dev, void *stream);
 
+/* Perfect sampling of every instance.  Replaces sample()'s noiseless path
+ * (simulator_mps_dist.cu:1587-1612) over sampleBitsGpu / sampleBitsDevice
+ * (sharded_mps.cuh:814-902):
+ *   seeds      : host array, one kernel seed per instance (the reference
+ *                passes hbatchSeedForShotShard(sampleSeed++, 0), :596-608)
+ *   bits_dev   : optional, instances*shots*n bytes 0/1, SITE order (site s =
+ *                qubit n-1-s), as k_chooseSample writes them
+ *   packed_dev : optional, instances*shots*2 uint64, bit s of word s/64 = site s */
+int gvmps_batch_sample(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
+                       uint8_t *bits_dev, uint64_t *packed_dev, void *stream);
+
+/* Finite-shot energies from one shared sample set per instance for all-Z/I
+ * Hamiltonians.  Replaces observe()'s diagonal shot path
+ * (simulator_mps_dist.cu:1493-1514, runSharedDiagonalShotBatch :1157-1171)
+ * over sampleZExpectationsGpu (sharded_mps.cuh:906-935).  Per-term values are
+ * (sum of parities)/shots; identity words give 1.  Fails for X/Y words. */
+int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
+                            double *energies_dev, double *terms_dev, void *stream);
+
 /* Per-phase device timing with HIP events recorded on the launch stream.
  * Phases: 0 state preparation, 1 identity environments, 2 prefix chains,
  * 3 energy reduction.  enable != 0 resets the accumulators.  No reference

@@ -263,7 +263,7 @@ class RefMPS:
             R[k] = np.einsum('aic,dic->ad', T, A.conj())         # k_rightFinish
         return R
 
-    def sample_bits(self, shots, seed, uniforms=None):
+    def sample_bits(self, shots, seed, uniforms=None, margins=False):
         """Perfect sampling with the GPU uniform stream.
 
         sampleBitsFromPreparedRight / k_sampleW / k_sampleProbabilities /
@@ -274,6 +274,7 @@ class RefMPS:
         n = self.n
         R = self.right_envs()
         bits = np.zeros((shots, n), dtype=np.int8)
+        margin = np.zeros((shots, n))
         v = np.ones((shots, 1), dtype=complex)
         for k in range(n):
             A = self.sites[k]
@@ -291,12 +292,13 @@ class RefMPS:
                     total = 1.0
                 u = rng.draw_uniform(seed, k, sh) if uniforms is None else uniforms[sh, k]
                 sel = 0 if u < p0 / total else 1
+                margin[sh, k] = abs(u - p0 / total)
                 ps = p1 if sel else p0
                 inv = 1.0 / np.sqrt(ps) if ps > 0.0 else 1.0
                 bits[sh, k] = sel
                 nxt[sh] = w[sh, sel] * inv
             v = nxt
-        return bits
+        return (bits, margin) if margins else bits
 
     def sample_strings(self, shots, seed):
         """sampleBitsGpu: qubit-order strings (q0 leftmost).  sharded_mps.cuh:887-902."""
