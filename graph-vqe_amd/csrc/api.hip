@@ -93,6 +93,12 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.nterms = P.nterms;
     d.nspill = P.nspill;
     d.prog = upload(B->owned, P.prog.data(), P.prog.size());
+    d.ngates = (int)P.g2.size();
+    d.nsched = (int)P.sched.size() / GV_WAVES;
+    d.g2 = upload(B->owned, P.g2.data(), P.g2.size());
+    d.sched = upload(B->owned, P.sched.data(), P.sched.size());
+    d.preops = upload(B->owned, P.preops.data(), P.preops.size());
+    d.flush = upload(B->owned, P.flush.data(), P.flush.size());
     d.gates = upload(B->owned, P.gates.data(), P.gates.size());
     d.fixes = upload(B->owned, P.fixes.data(), P.fixes.size());
     d.chains = upload(B->owned, P.chains.data(), P.chains.size());
@@ -156,6 +162,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->terms = upload<double2>(B->owned, nullptr, term);
   B->status = upload<int>(B->owned, nullptr, B->inst.size());
   B->bonds = upload<int>(B->owned, nullptr, B->inst.size() * (GV_NMAX + 1));
+  B->stats = upload<int>(B->owned, nullptr, B->inst.size() * GV_STATS);
   B->energies = upload<double>(B->owned, nullptr, B->inst.size());
   B->seeds = upload<unsigned>(B->owned, nullptr, B->inst.size());
   // chain work items per generation, instance-major
@@ -193,7 +200,8 @@ void check_status(Batch &B, hipStream_t stream) {
 void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
   const int ninst = (int)B.inst.size();
   B.mark(0, stream);
-  hip_check(kl_prep(ninst, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status), "k_prep launch");
+  hip_check(kl_prep(ninst, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status, B.stats),
+            "k_prep launch");
   B.mark(1, stream);
 }
 
@@ -305,6 +313,8 @@ int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
     info->n_spill = P.nspill;
     info->n_generations = P.ngen;
     info->n_child_nodes = P.n_nodes;
+    info->n_right_nodes = P.n_right_nodes;
+    info->n_sched_steps = (int)P.sched.size() / GV_WAVES;
     info->flops_energy_executed = P.flops_exec;
     info->flops_energy_nodes = P.flops_nodes;
     info->flops_svd_nominal = P.flops_svd;
@@ -403,6 +413,16 @@ int gvmps_batch_get_timing(gvmps_batch *b, double *phase_ms, long long *calls) {
     Batch &B = *reinterpret_cast<Batch *>(b);
     for (int i = 0; i < 8; ++i) phase_ms[i] = B.phase_ms[i];
     if (calls) *calls = B.phase_calls;
+  })
+}
+
+int gvmps_batch_prep_stats(gvmps_batch *b, int *sweeps_host) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hip_check(hipMemcpy(sweeps_host, B.stats, sizeof(int) * B.inst.size() * GV_STATS, hipMemcpyDeviceToHost),
+              "read stats");
   })
 }
 

@@ -21,7 +21,7 @@ struct Batch {
   ProbDev *d_probs = nullptr;
   InstDev *d_inst = nullptr;
   double2 *arena = nullptr, *envs = nullptr, *spills = nullptr, *terms = nullptr;
-  int *status = nullptr, *bonds = nullptr;
+  int *status = nullptr, *bonds = nullptr, *stats = nullptr;
   double *energies = nullptr;
   // sampling
   const unsigned long long **d_masks = nullptr;  // per problem: 2 words per term (site order)
@@ -55,7 +55,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream);
 
 // kernel launchers (kernels_energy.hip)
 hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, const double *params,
-                   double2 *arena, int *bonds, int *status);
+                   double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
 hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                      const double2 *arena, const double2 *envs, double2 *spills, double2 *terms);

@@ -30,14 +30,33 @@ struct ProgOp {
   double offset;
 };
 
+// Two-site gates of the pipelined state-preparation schedule.  The single-
+// site ops that precede a gate on its two sites (in program order) are folded
+// into it: pre-ops [pre0, pre0+npre_l) act on the left site, the next npre_r
+// on the right site.
+#define GV_WAVES 4           // gates in flight per instance (one wavefront each)
+#define GV_STATS 8           // state-preparation statistics per instance (ints)
+struct GateDev {
+  int site;    // left site s of (s, s+1)
+  int gate;    // 4x4 gate table index
+  int pre0, npre_l, npre_r;
+};
+
 // Children kinds of a prefix node expansion.
 enum ChildKind : int { CK_I = 0, CK_Z = 1, CK_X = 2, CK_Y = 3 };
 
+// The prefix forest grows terms from their first non-identity site to the
+// right (dir 0, left environments, A_i); the suffix forest grows them from
+// their last non-identity site to the left (dir 1, transposed environments,
+// A_i^T).  Each term is split once: its left part ends at site c, its right
+// part starts at c+1 and <P> = Tr(E_left(c) R_right(c+1)).
 struct ChainDev {      // one chain of expansions
   int site0;           // site of the first expansion
   int nsteps;
   int step0;           // index of the first step
-  int src;             // -1: start from L[site0]; >= 0: spill slot holding the start env
+  int src;             // -1: start from the identity environment; >= 0: spill slot
+  int dir;             // 0: prefix forest (left to right), 1: suffix forest (right to left)
+  int pad;
 };
 
 struct StepDev {       // one expansion: E -> children at site `site`
@@ -50,8 +69,17 @@ struct StepDev {       // one expansion: E -> children at site `site`
 struct ChildDev {
   int kind;            // ChildKind
   int action;          // -1: none, -2: continue the chain, >= 0: spill slot
-  int term0;           // range into the term-end list
+  int term0;           // range into the term-end list (prefix forest)
   int nterm;
+  int store;           // suffix forest: slot receiving this node's environment, -1 none
+  int pad;
+};
+
+// A term ending at a prefix-forest node: its index and the slot of its
+// suffix-forest environment (-1: identity right environment Rt[site+1]).
+struct TermEnd {
+  int term;
+  int rslot;
 };
 
 // Device view of one compiled problem (circuit + Hamiltonian).
@@ -64,12 +92,17 @@ struct ProbDev {
   int nterms;
   int nspill;
   const ProgOp* prog;
+  int ngates, nsched;      // two-site gates; schedule rows of GV_WAVES gate slots
+  const GateDev* g2;
+  const int* sched;        // nsched * GV_WAVES gate indices, -1 = idle slot
+  const ProgOp* preops;
+  const int* flush;        // n+1 offsets into preops: ops left pending on site s at the end
   const double* gates;     // 2q gate table, 16 complex each (interleaved re,im)
   const double* fixes;     // 1q fixed table, 4 complex each
   const ChainDev* chains;
   const StepDev* steps;
   const ChildDev* children;
-  const int* term_list;
+  const TermEnd* term_list;
   const double* coeffs;    // nterms complex (re,im)
   const int* active;       // nterms flags: 1 = contracted, 0 = identity word (<P> = 1)
 };

@@ -190,103 +190,213 @@ struct CircuitCompiler {
   }
 };
 
+// Pipelined gate schedule.  A two-site gate depends on the previous gates on
+// either of its sites; gates at the same ASAP level touch disjoint sites and
+// run concurrently, GV_WAVES per time step.  The order of operations on every
+// single site is the program order, so the state is the same as the
+// sequential gate stream of the reference (simulator_mps_dist.cu:1392-1406).
+void build_schedule(Problem &P) {
+  const int n = P.n;
+  std::vector<std::vector<ProgOp>> pending(n);
+  std::vector<int> last(n, -1), level;
+  P.g2.clear();
+  P.preops.clear();
+  for (const ProgOp &op : P.prog) {
+    if (op.kind != OP_G2) {
+      pending[op.site].push_back(op);
+      continue;
+    }
+    const int s = op.site;
+    GateDev gd{};
+    gd.site = s;
+    gd.gate = op.arg;
+    gd.pre0 = (int)P.preops.size();
+    gd.npre_l = (int)pending[s].size();
+    gd.npre_r = (int)pending[s + 1].size();
+    P.preops.insert(P.preops.end(), pending[s].begin(), pending[s].end());
+    P.preops.insert(P.preops.end(), pending[s + 1].begin(), pending[s + 1].end());
+    pending[s].clear();
+    pending[s + 1].clear();
+    int lv = 0;
+    if (last[s] >= 0) lv = std::max(lv, level[last[s]] + 1);
+    if (last[s + 1] >= 0) lv = std::max(lv, level[last[s + 1]] + 1);
+    level.push_back(lv);
+    last[s] = last[s + 1] = (int)P.g2.size();
+    P.g2.push_back(gd);
+  }
+  P.flush.assign(n + 1, 0);
+  for (int s = 0; s < n; ++s) {
+    P.flush[s] = (int)P.preops.size();
+    P.preops.insert(P.preops.end(), pending[s].begin(), pending[s].end());
+  }
+  P.flush[n] = (int)P.preops.size();
+  int maxlv = -1;
+  for (int lv : level) maxlv = std::max(maxlv, lv);
+  std::vector<std::vector<int>> by_level(maxlv + 1);
+  for (int g = 0; g < (int)level.size(); ++g) by_level[level[g]].push_back(g);
+  P.sched.clear();
+  for (const auto &gs : by_level)
+    for (size_t i = 0; i < gs.size(); i += GV_WAVES)
+      for (int w = 0; w < GV_WAVES; ++w)
+        P.sched.push_back(i + w < gs.size() ? gs[i + w] : -1);
+}
+
 struct TNode {
   int site, kind, parent;
   int child[4];
-  std::vector<int> terms;
-  long long work = 0;
+  std::vector<TermEnd> ends;  // prefix forest: terms whose left part ends here
+  int store = -1;             // suffix forest: slot receiving this node's environment
+  int refs = 0;               // split planning: terms whose path uses this node
+  long long work = 0;         // expansions in the subtree (heavy-path choice)
 };
 
-void build_chains(Problem &P) {
-  const int n = P.n;
+// A trie of Pauli paths grown from each start site in one direction.
+struct Forest {
+  int dir;  // 0: sites increase (prefix forest), 1: sites decrease (suffix forest)
   std::vector<TNode> nodes;
-  std::vector<std::array<int, 4>> roots(n);
-  for (auto &r : roots) r = {-1, -1, -1, -1};
-  auto make = [&](int site, int kind, int parent) {
+  std::vector<std::array<int, 4>> roots;
+  Forest(int n, int d) : dir(d), roots(n) {
+    for (auto &r : roots) r = {-1, -1, -1, -1};
+  }
+  int next(int site) const { return dir == 0 ? site + 1 : site - 1; }
+  int find(int parent, int site, int kind) const {
+    return parent < 0 ? roots[site][kind] : nodes[parent].child[kind];
+  }
+  int get(int parent, int site, int kind) {
+    int c = find(parent, site, kind);
+    if (c >= 0) return c;
     TNode t;
     t.site = site;
     t.kind = kind;
     t.parent = parent;
-    for (int &c : t.child) c = -1;
+    for (int &x : t.child) x = -1;
     nodes.push_back(std::move(t));
-    return (int)nodes.size() - 1;
-  };
+    c = (int)nodes.size() - 1;
+    if (parent < 0) roots[site][kind] = c;
+    else nodes[parent].child[kind] = c;
+    return c;
+  }
+  // Path of word w from `from` to `to` (inclusive, in this forest's direction);
+  // adds `delta` to every node's refcount; returns the last node.
+  int walk(const std::string &w, int from, int to, int delta) {
+    int node = -1;
+    for (int s = from;; s = next(s)) {
+      node = get(node, s, kind_of(w[s]));
+      nodes[node].refs += delta;
+      if (s == to) break;
+    }
+    return node;
+  }
+  // First position (walking from `from` towards `limit`) whose node is
+  // absent or unused; returns next(limit) when the whole path exists.
+  int first_new(const std::string &w, int from, int limit) const {
+    int node = -1;
+    for (int s = from;; s = next(s)) {
+      node = find(node, s, kind_of(w[s]));
+      if (node < 0 || nodes[node].refs <= 0) return s;
+      if (s == limit) return next(s);
+    }
+  }
+};
+
+struct TermPath {
+  int k, first, last, c;
+};
+
+// Split choice: term k's left part is [first, c] in the prefix forest, its
+// right part [c+1, last] in the suffix forest.  The number of new nodes for a
+// split c is max(0, c - p0 + 1) + max(0, q - c) where the prefix forest
+// already holds the path up to p0 - 1 and the suffix forest down to q + 1.
+// Greedy: start every term at its middle non-identity site, then re-place
+// each term once at its cheapest split (ties: nearest the middle).
+std::vector<TermPath> choose_splits(const Problem &P) {
+  const int n = P.n;
+  std::vector<TermPath> paths;
   for (int k : P.active) {
     const std::string &w = P.site_words[k];
-    int a = 0, b = n - 1;
-    while (w[a] == 'I') ++a;
-    while (w[b] == 'I') --b;
-    int kd = kind_of(w[a]);
-    int node = roots[a][kd];
-    if (node < 0) node = roots[a][kd] = make(a, kd, -1);
-    for (int s = a + 1; s <= b; ++s) {
-      kd = kind_of(w[s]);
-      int nx = nodes[node].child[kd];
-      if (nx < 0) {
-        nx = make(s, kd, node);
-        nodes[node].child[kd] = nx;
-      }
-      node = nx;
-    }
-    nodes[node].terms.push_back(k);
+    std::vector<int> z;
+    for (int s = 0; s < n; ++s)
+      if (w[s] != 'I') z.push_back(s);
+    paths.push_back({k, z.front(), z.back(), z[(z.size() - 1) / 2]});
   }
-  P.n_nodes = (int)nodes.size();
-  // Subtree expansion counts (children are created after their parents).
-  for (int v = (int)nodes.size() - 1; v >= 0; --v) {
+  Forest L(n, 0), R(n, 1);
+  auto insert = [&](const TermPath &t, int delta) {
+    const std::string &w = P.site_words[t.k];
+    L.walk(w, t.first, t.c, delta);
+    if (t.c < t.last) R.walk(w, t.last, t.c + 1, delta);
+  };
+  for (const TermPath &t : paths) insert(t, +1);
+  for (TermPath &t : paths) {
+    insert(t, -1);
+    const std::string &w = P.site_words[t.k];
+    const int p0 = L.first_new(w, t.first, t.last);
+    const int q = R.first_new(w, t.last, t.first);
+    const int mid = t.c;
+    int best = -1, bc = mid;
+    for (int c = t.first; c <= t.last; ++c) {
+      const int cost = std::max(0, c - p0 + 1) + std::max(0, q - c);
+      if (best < 0 || cost < best || (cost == best && std::abs(c - mid) < std::abs(bc - mid))) {
+        best = cost;
+        bc = c;
+      }
+    }
+    t.c = bc;
+    insert(t, +1);
+  }
+  return paths;
+}
+
+struct ChainOut {
+  ChainDev ch;
+  int gen;
+  std::vector<StepDev> steps;
+  std::vector<ChildDev> kids;  // term0 temporarily holds the node id
+};
+
+// Heavy-path chain decomposition of one forest.  Each chain keeps its
+// environment in registers; a child that starts another chain is spilled and
+// that chain belongs to the next generation.
+void forest_chains(Forest &F, int n, int gen0, int &nspill, double &flops, std::vector<ChainOut> &out) {
+  for (int v = (int)F.nodes.size() - 1; v >= 0; --v) {
     long long w = 0;
     bool any = false;
-    for (int c : nodes[v].child)
+    for (int c : F.nodes[v].child)
       if (c >= 0) {
         any = true;
-        w += nodes[c].work;
+        w += F.nodes[c].work;
       }
-    nodes[v].work = w + (any ? 1 : 0);
+    F.nodes[v].work = w + (any ? 1 : 0);
   }
-  // Algorithmic node count: 16 (Dl^2 Dr + Dl Dr^2) flops per prefix node.
-  const std::vector<int> &bd = P.bonds_pred;
-  auto fsite = [&](int s) {
-    const double dl = bd[s], dr = bd[s + 1];
-    return 16.0 * (dl * dl * dr + dl * dr * dr);
-  };
-  P.flops_nodes = 0;
-  for (const TNode &t : nodes) P.flops_nodes += fsite(t.site);
-  for (int s = 0; s < n; ++s) P.flops_nodes += 2.0 * fsite(s);
-
   struct Start {
     int slot, site, node, gen;
   };
   std::vector<Start> queue;
-  for (int a = 0; a < n; ++a)
-    if (roots[a][0] >= 0 || roots[a][1] >= 0 || roots[a][2] >= 0 || roots[a][3] >= 0)
-      queue.push_back({-1, a, -1, 0});
-  std::vector<ChainDev> chains;
-  std::vector<int> gens;
-  std::vector<std::vector<StepDev>> chain_steps;
-  std::vector<std::vector<ChildDev>> chain_children;
-  P.nspill = 0;
+  for (int a = 0; a < n; ++a) {
+    const auto &r = F.roots[a];
+    if (r[0] >= 0 || r[1] >= 0 || r[2] >= 0 || r[3] >= 0) queue.push_back({-1, a, -1, gen0});
+  }
   const double chi = GV_CHI;
-  P.flops_exec = 2.0 * n * 32.0 * chi * chi * chi;  // L and R identity sweeps, 4 chi^3 complex MACs each
   for (size_t qi = 0; qi < queue.size(); ++qi) {
     const Start st = queue[qi];
-    ChainDev ch{};
-    ch.site0 = st.site;
-    ch.src = st.slot;
-    std::vector<StepDev> steps;
-    std::vector<ChildDev> kids_out;
-    std::array<int, 4> kids = st.node < 0 ? roots[st.site] : std::array<int, 4>{nodes[st.node].child[0],
-                                                                               nodes[st.node].child[1],
-                                                                               nodes[st.node].child[2],
-                                                                               nodes[st.node].child[3]};
+    ChainOut co;
+    co.ch = ChainDev{};
+    co.ch.site0 = st.site;
+    co.ch.src = st.slot;
+    co.ch.dir = F.dir;
+    co.gen = st.gen;
+    std::array<int, 4> kids = st.node < 0 ? F.roots[st.site]
+                                          : std::array<int, 4>{F.nodes[st.node].child[0], F.nodes[st.node].child[1],
+                                                               F.nodes[st.node].child[2], F.nodes[st.node].child[3]};
     int site = st.site;
     for (;;) {
       StepDev sd{};
       sd.site = site;
-      sd.child0 = (int)kids_out.size();
+      sd.child0 = (int)co.kids.size();
       int cont = -1;
       long long best = 0;
       for (int c : kids)
-        if (c >= 0 && nodes[c].work > best) {
-          best = nodes[c].work;
+        if (c >= 0 && F.nodes[c].work > best) {
+          best = F.nodes[c].work;
           cont = c;
         }
       for (int kd = 0; kd < 4; ++kd) {
@@ -295,37 +405,77 @@ void build_chains(Problem &P) {
         ChildDev cdv{};
         cdv.kind = kd;
         if (c == cont) cdv.action = -2;
-        else if (nodes[c].work > 0) {
-          cdv.action = P.nspill++;
-          queue.push_back({cdv.action, site + 1, c, st.gen + 1});
+        else if (F.nodes[c].work > 0) {
+          cdv.action = nspill++;
+          queue.push_back({cdv.action, F.next(site), c, st.gen + 1});
         } else
           cdv.action = -1;
-        cdv.term0 = -1;  // filled when flattening
-        cdv.nterm = (int)nodes[c].terms.size();
-        cdv.term0 = c;   // temporarily: node id
-        kids_out.push_back(cdv);
-        if (kd <= CK_Z) sd.need |= 1;
-        else sd.need |= 2;
+        cdv.nterm = (int)F.nodes[c].ends.size();
+        cdv.store = F.nodes[c].store;
+        cdv.term0 = c;  // node id until flattened
+        co.kids.push_back(cdv);
+        sd.need |= (kd <= CK_Z) ? 1 : 2;
       }
-      sd.nchild = (int)kids_out.size() - sd.child0;
-      P.flops_exec += (sd.need == 3 ? 48.0 : 32.0) * chi * chi * chi;  // (4 or 6) chi^3 complex MACs
-      steps.push_back(sd);
+      sd.nchild = (int)co.kids.size() - sd.child0;
+      flops += (sd.need == 3 ? 48.0 : 32.0) * chi * chi * chi;  // (4 or 6) chi^3 complex MACs
+      co.steps.push_back(sd);
       if (cont < 0) break;
-      kids = {nodes[cont].child[0], nodes[cont].child[1], nodes[cont].child[2], nodes[cont].child[3]};
-      ++site;
+      kids = {F.nodes[cont].child[0], F.nodes[cont].child[1], F.nodes[cont].child[2], F.nodes[cont].child[3]};
+      site = F.next(site);
     }
-    ch.nsteps = (int)steps.size();
-    chains.push_back(ch);
-    gens.push_back(st.gen);
-    chain_steps.push_back(std::move(steps));
-    chain_children.push_back(std::move(kids_out));
+    co.ch.nsteps = (int)co.steps.size();
+    out.push_back(std::move(co));
   }
-  // Order: generation ascending, then longest chains first.
+}
+
+void build_chains(Problem &P) {
+  const int n = P.n;
+  const std::vector<TermPath> paths = choose_splits(P);
+  Forest L(n, 0), R(n, 1);
+  P.nspill = 0;
+  std::vector<int> right_leaf(paths.size(), -1);
+  for (size_t i = 0; i < paths.size(); ++i) {
+    const TermPath &t = paths[i];
+    const std::string &w = P.site_words[t.k];
+    if (t.c < t.last) {
+      const int rn = R.walk(w, t.last, t.c + 1, +1);
+      if (R.nodes[rn].store < 0) R.nodes[rn].store = P.nspill++;
+      right_leaf[i] = R.nodes[rn].store;
+    }
+  }
+  for (size_t i = 0; i < paths.size(); ++i) {
+    const TermPath &t = paths[i];
+    const int ln = L.walk(P.site_words[t.k], t.first, t.c, +1);
+    L.nodes[ln].ends.push_back(TermEnd{t.k, right_leaf[i]});
+  }
+  P.n_nodes = (int)L.nodes.size();
+  P.n_right_nodes = (int)R.nodes.size();
+  // Algorithmic node count: 16 (Dl^2 Dr + Dl Dr^2) flops per forest node,
+  // plus the two identity sweeps.
+  const std::vector<int> &bd = P.bonds_pred;
+  auto fsite = [&](int s) {
+    const double dl = bd[s], dr = bd[s + 1];
+    return 16.0 * (dl * dl * dr + dl * dr * dr);
+  };
+  P.flops_nodes = 0;
+  for (const TNode &t : L.nodes) P.flops_nodes += fsite(t.site);
+  for (const TNode &t : R.nodes) P.flops_nodes += fsite(t.site);
+  for (int s = 0; s < n; ++s) P.flops_nodes += 2.0 * fsite(s);
+
+  const double chi = GV_CHI;
+  P.flops_exec = 2.0 * n * 32.0 * chi * chi * chi;  // identity sweeps, 4 chi^3 complex MACs each
+  std::vector<ChainOut> chains;
+  forest_chains(R, n, 0, P.nspill, P.flops_exec, chains);
+  int right_gens = 0;
+  for (const ChainOut &c : chains) right_gens = std::max(right_gens, c.gen + 1);
+  const size_t nright = chains.size();
+  forest_chains(L, n, right_gens, P.nspill, P.flops_exec, chains);
+  // Order: generation ascending (suffix forest first), longest chains first.
   std::vector<int> order(chains.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    if (gens[x] != gens[y]) return gens[x] < gens[y];
-    return chains[x].nsteps > chains[y].nsteps;
+    if (chains[x].gen != chains[y].gen) return chains[x].gen < chains[y].gen;
+    return chains[x].ch.nsteps > chains[y].ch.nsteps;
   });
   P.chains.clear();
   P.steps.clear();
@@ -334,19 +484,21 @@ void build_chains(Problem &P) {
   P.gen_begin.clear();
   int last_gen = -1;
   for (int idx : order) {
-    while (last_gen < gens[idx]) {
+    const ChainOut &co = chains[idx];
+    while (last_gen < co.gen) {
       P.gen_begin.push_back((int)P.chains.size());
       ++last_gen;
     }
-    ChainDev ch = chains[idx];
+    const Forest &F = (size_t)idx < nright ? R : L;
+    ChainDev ch = co.ch;
     ch.step0 = (int)P.steps.size();
-    for (StepDev sd : chain_steps[idx]) {
+    for (StepDev sd : co.steps) {
       const int base = (int)P.children.size();
       for (int c = 0; c < sd.nchild; ++c) {
-        ChildDev cdv = chain_children[idx][sd.child0 + c];
+        ChildDev cdv = co.kids[sd.child0 + c];
         const int node = cdv.term0;
         cdv.term0 = (int)P.term_list.size();
-        for (int t : nodes[node].terms) P.term_list.push_back(t);
+        for (const TermEnd &te : F.nodes[node].ends) P.term_list.push_back(te);
         P.children.push_back(cdv);
       }
       sd.child0 = base;
@@ -378,6 +530,7 @@ std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int 
   if (P->max_alloc_bond > GV_CHI)
     throw std::runtime_error("allocated bond " + std::to_string(P->max_alloc_bond) +
                              " exceeds the 16-wide bond slots of the GPU layout");
+  build_schedule(*P);
   P->nterms = n_terms;
   P->coeffs.assign(coeffs, coeffs + 2 * (size_t)n_terms);
   P->site_words.resize(n_terms);

@@ -22,6 +22,11 @@ struct Problem {
   std::vector<double> fixes;   // 8 doubles per fixed single-site gate
   std::vector<int> bonds_pred; // n+1, zero-cutoff allocation
   int max_alloc_bond = 1, n_g2 = 0;
+  // pipelined schedule of the two-site gates
+  std::vector<GateDev> g2;
+  std::vector<int> sched;      // rows of GV_WAVES gate indices (-1 idle)
+  std::vector<ProgOp> preops;  // gate pre-ops, then the final per-site flush ops
+  std::vector<int> flush;      // n+1 offsets into preops
   double flops_svd = 0;
   // Hamiltonian
   int nterms = 0;
@@ -32,7 +37,8 @@ struct Problem {
   std::vector<ChainDev> chains;
   std::vector<StepDev> steps;
   std::vector<ChildDev> children;
-  std::vector<int> term_list;
+  std::vector<TermEnd> term_list;
+  int n_right_nodes = 0;                 // suffix-forest nodes
   std::vector<int> gen_begin;            // chains of generation g: [gen_begin[g], gen_begin[g+1])
   int nspill = 0, ngen = 0, n_nodes = 0;
   double flops_exec = 0, flops_nodes = 0;

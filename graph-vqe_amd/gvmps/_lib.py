@@ -17,7 +17,8 @@ class Op(ctypes.Structure):
 class ProblemInfo(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in (
         'n_qubits', 'n_params', 'n_terms', 'n_active_terms', 'n_prog_ops', 'n_two_site_gates',
-        'max_alloc_bond', 'n_chains', 'n_steps', 'n_spill', 'n_generations', 'n_child_nodes')] + [
+        'max_alloc_bond', 'n_chains', 'n_steps', 'n_spill', 'n_generations', 'n_child_nodes',
+        'n_sched_steps', 'n_right_nodes')] + [
         ('flops_energy_executed', ctypes.c_double), ('flops_energy_nodes', ctypes.c_double),
         ('flops_svd_nominal', ctypes.c_double)]
 
@@ -41,6 +42,7 @@ _PROTOS = {
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_set_timing': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_get_timing': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
+    'gvmps_batch_prep_stats': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     'gvmps_batch_read_sites': [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_int)],
 }

@@ -175,6 +175,12 @@ class Batch:
         check(lib().gvmps_batch_get_timing(self._h, ms, ctypes.byref(calls)))
         return list(ms), calls.value
 
+    def prep_stats(self):
+        """Per instance: [sweeps, build, qr, jacobi, tail clocks/16, ...] of the last preparation."""
+        out = (ctypes.c_int * (8 * self.size))()
+        check(lib().gvmps_batch_prep_stats(self._h, out))
+        return np.array(out, dtype=np.int64).reshape(self.size, 8)
+
     def read_sites(self, inst):
         """Prepared site tensors of instance `inst` as numpy (n, 16, 2, 16) + bonds."""
         j, k = 0, inst

@@ -45,7 +45,9 @@ typedef struct gvmps_problem_info {
   int32_t n_prog_ops, n_two_site_gates;
   int32_t max_alloc_bond;          /* predicted allocated bond (zero-cutoff rule) */
   int32_t n_chains, n_steps, n_spill, n_generations;
-  int32_t n_child_nodes;           /* distinct prefix nodes (spans only) */
+  int32_t n_child_nodes;           /* prefix-forest nodes */
+  int32_t n_sched_steps;           /* time steps of the pipelined gate schedule */
+  int32_t n_right_nodes;           /* suffix-forest nodes */
   double flops_energy_executed;    /* FP64 flops of one energy eval, as executed */
   double flops_energy_nodes;       /* prefix-node algorithmic count, 16(Dl^2Dr+DlDr^2) per node */
   double flops_svd_nominal;        /* Golub-Reinsch nominal count of all SVDs */
@@ -118,6 +120,13 @@ int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots,
  * test/tools/analytic_execution.py:53-62); used by the benchmark. */
 int gvmps_batch_set_timing(gvmps_batch *b, int enable);
 int gvmps_batch_get_timing(gvmps_batch *b, double *phase_ms /* 8 */, long long *calls);
+
+/* Statistics of each instance's last state preparation, 8 ints per instance:
+ * Jacobi sweeps summed over its two-site gates, then the wavefront clock
+ * counts (in units of 16 clocks, summed over gates) of the Theta build, the
+ * pivoted QR, the Jacobi sweeps and the rank rule + write-back.  Diagnostic;
+ * no reference counterpart. */
+int gvmps_batch_prep_stats(gvmps_batch *b, int *stats_host);
 
 /* Read back the prepared site tensors of instance `inst` (host buffer of
  * n*16*2*16 complex) and its bond dimensions (n+1 ints).  Test/debug aid. */
