@@ -92,6 +92,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.rel_cut = P.rel_cut;
     d.nterms = P.nterms;
     d.nspill = P.nspill;
+    d.nchains = (int)P.chains.size();
     d.prog = upload(B->owned, P.prog.data(), P.prog.size());
     d.ngates = (int)P.g2.size();
     d.nsched = (int)P.sched.size() / GV_WAVES;
@@ -129,7 +130,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   }
   B->d_masks = upload(B->owned, mask_ptrs.data(), mask_ptrs.size());
   // instances
-  long long site = 0, env = 0, spill = 0, term = 0, param = 0;
+  long long site = 0, env = 0, spill = 0, term = 0, param = 0, part = 0;
   for (int j = 0; j < n_problems; ++j) {
     const Problem &P = *problems[j]->p;
     if (counts[j] < 0) throw std::runtime_error("negative instance count");
@@ -142,6 +143,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       in.spill_off = spill;
       in.term_off = term;
       in.param_off = param;
+      in.part_off = part;
       in.out = (int)B->inst.size();
       B->inst.push_back(in);
       site += (long long)P.n * GV_SLOT;
@@ -149,6 +151,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       spill += (long long)P.nspill * GV_ENV;
       term += P.nterms;
       param += P.nparams;
+      part += (long long)P.chains.size();
       B->max_gen = std::max(B->max_gen, P.ngen);
     }
   }
@@ -165,6 +168,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->stats = upload<int>(B->owned, nullptr, B->inst.size() * GV_STATS);
   B->energies = upload<double>(B->owned, nullptr, B->inst.size());
   B->seeds = upload<unsigned>(B->owned, nullptr, B->inst.size());
+  B->parts = upload<double>(B->owned, nullptr, part);
   // chain work items per generation, instance-major
   for (int g = 0; g < B->max_gen; ++g) {
     std::vector<int2> items;
@@ -205,18 +209,24 @@ void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
   B.mark(1, stream);
 }
 
-void launch_energy_tail(Batch &B, hipStream_t stream) {
+// terms == true: per-term <P_k> into B.terms (observe's per-term data);
+// otherwise chain partial energies only.
+void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
   const int ninst = (int)B.inst.size();
-  hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
   B.mark(2, stream);
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains(ni, stream, B.items[g], B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.terms),
-              "k_chains launch");
+    if (terms)
+      hip_check(kl_chains(ni, stream, B.items[g], B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.terms),
+                "k_chains launch");
+    else
+      hip_check(kl_chains_energy(ni, stream, B.items[g], B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.parts),
+                "k_chains_energy launch");
   }
   B.mark(3, stream);
-  hip_check(kl_reduce(ninst, stream, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
+  if (terms) hip_check(kl_reduce(ninst, stream, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
+  else hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies), "k_reduce launch");
   B.mark(4, stream);
 }
 
@@ -296,6 +306,19 @@ int gvmps_problem_destroy(gvmps_problem *p) {
   })
 }
 
+int gvmps_problem_get_generations(const gvmps_problem *p, int *out, int max_gens) {
+  GV_TRY({
+    if (!p || !out) throw std::runtime_error("null pointer");
+    const Problem &P = *p->p;
+    for (int g = 0; g < P.ngen && g < max_gens; ++g) {
+      int steps = 0;
+      for (int c = P.gen_begin[g]; c < P.gen_begin[g + 1]; ++c) steps += P.chains[c].nsteps;
+      out[2 * g] = P.gen_begin[g + 1] - P.gen_begin[g];
+      out[2 * g + 1] = steps;
+    }
+  })
+}
+
 int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
   GV_TRY({
     if (!p || !info) throw std::runtime_error("null pointer");
@@ -351,7 +374,7 @@ int gvmps_batch_energy(gvmps_batch *b, const double *params_dev, double *energie
     hipStream_t s = (hipStream_t)stream;
     hip_check(hipSetDevice(B.device), "hipSetDevice");
     launch_prepare(B, params_dev, s);
-    launch_energy_tail(B, s);
+    launch_energy_tail(B, s, terms_dev != nullptr);
     if (energies_dev)
       hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * B.inst.size(),
                                hipMemcpyDeviceToDevice, s),

@@ -21,6 +21,7 @@ struct Batch {
   ProbDev *d_probs = nullptr;
   InstDev *d_inst = nullptr;
   double2 *arena = nullptr, *envs = nullptr, *spills = nullptr, *terms = nullptr;
+  double *parts = nullptr;  // chain partial energies
   int *status = nullptr, *bonds = nullptr, *stats = nullptr;
   double *energies = nullptr;
   // sampling
@@ -51,7 +52,7 @@ void hip_check(hipError_t e, const char *what);
 Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_problems, int device);
 void check_status(Batch &B, hipStream_t stream);
 void launch_prepare(Batch &B, const double *params, hipStream_t stream);
-void launch_energy_tail(Batch &B, hipStream_t stream);
+void launch_energy_tail(Batch &B, hipStream_t stream, bool terms);
 
 // kernel launchers (kernels_energy.hip)
 hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, const double *params,
@@ -61,6 +62,11 @@ hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev
                      const double2 *arena, const double2 *envs, double2 *spills, double2 *terms);
 hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
                      double *energies);
+hipError_t kl_chains_energy(int nitems, hipStream_t s, const int2 *items, const InstDev *insts,
+                            const ProbDev *probs, const double2 *arena, const double2 *envs, double2 *spills,
+                            double *parts);
+hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                            const double *parts, double *energies);
 // kernel launchers (kernels_sample.hip)
 hipError_t kl_sample(int ninst, int shots, int stride, hipStream_t s, const InstDev *insts, const double2 *arena,
                      const double2 *envs, const unsigned *seeds, unsigned char *bits, unsigned long long *packed);

@@ -42,6 +42,23 @@ struct GateDev {
   int pre0, npre_l, npre_r;
 };
 
+// Bond-doubling construction of one site for the reverse gradient (no SVD):
+// a CNOT on neighbouring sites doubles their shared bond exactly
+// (test/tools/reverse_mps.py:45-50, 161-171).
+enum LocalKind : int {
+  LO_ROT = 1,     // rotation, arg = axis, param/scale/offset as ProgOp, rot = rotation id
+  LO_FIX = 2,     // fixed 2x2, arg = fix table index
+  LO_CTRL_R = 3,  // control, partner on the right: A'[l,i,(r,s)] = A[l,i,r] d(i,s)
+  LO_CTRL_L = 4,  // control, partner on the left:  A'[(l,s),i,r] = A[l,i,r] d(i,s)
+  LO_TARG_R = 5,  // target, partner on the right:  A'[l,i,(r,s)] = A[l,i^s,r]
+  LO_TARG_L = 6   // target, partner on the left:   A'[(l,s),i,r] = A[l,i^s,r]
+};
+struct LocalOp {
+  int kind, arg, param, rot;
+  double scale, offset;
+  int dl, dr;  // bond dimensions after the op
+};
+
 // Children kinds of a prefix node expansion.
 enum ChildKind : int { CK_I = 0, CK_Z = 1, CK_X = 2, CK_Y = 3 };
 
@@ -72,7 +89,7 @@ struct ChildDev {
   int term0;           // range into the term-end list (prefix forest)
   int nterm;
   int store;           // suffix forest: slot receiving this node's environment, -1 none
-  int pad;
+  int gleaf;           // gradient: slot keeping this prefix leaf's environment, -1 none
 };
 
 // A term ending at a prefix-forest node: its index and the slot of its
@@ -80,6 +97,13 @@ struct ChildDev {
 struct TermEnd {
   int term;
   int rslot;
+};
+
+// A term whose suffix-forest environment sits in a store slot; its prefix
+// leaf environment was kept in gradient slot `gleaf` (reverse gradient seeds).
+struct SeedTerm {
+  int term;
+  int gleaf;
 };
 
 // Device view of one compiled problem (circuit + Hamiltonian).
@@ -91,6 +115,7 @@ struct ProbDev {
   double abs_cut, rel_cut;
   int nterms;
   int nspill;
+  int nchains;
   const ProgOp* prog;
   int ngates, nsched;      // two-site gates; schedule rows of GV_WAVES gate slots
   const GateDev* g2;
@@ -105,6 +130,16 @@ struct ProbDev {
   const TermEnd* term_list;
   const double* coeffs;    // nterms complex (re,im)
   const int* active;       // nterms flags: 1 = contracted, 0 = identity word (<P> = 1)
+  // reverse gradient (bond-doubling state)
+  int nsteps, ngleaf, nrot;
+  const LocalOp* lops;     // per-site local programs
+  const int* lop_begin;    // n+1 offsets
+  const int* seed_begin;   // nspill+1 offsets: terms seeding each suffix-forest store slot
+  const SeedTerm* seed_terms;
+  const int* site_steps_begin;  // n+1 offsets
+  const int* site_steps;   // steps at each site (fixed summation order)
+  const int* rot_param;    // per rotation: parameter index (-1 constant)
+  const double* rot_scale;
 };
 
 // One MPS instance of a batch.
@@ -116,6 +151,7 @@ struct InstDev {
   long long spill_off;
   long long term_off;  // in complex elements
   long long param_off; // in doubles
+  long long part_off;  // chain partial energies (one double per chain)
   int out;             // output index (energy slot)
   int pad;
 };

@@ -48,11 +48,16 @@ k_envs(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict__
   }
 }
 
-// One generation of prefix chains.  Work item (instance, chain) per wave.
-extern "C" __global__ void __launch_bounds__(256)
-k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-         const ProbDev *__restrict__ probs, const double2 *__restrict__ arena,
-         const double2 *__restrict__ envs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+// One generation of prefix/suffix chains.  Work item (instance, chain) per
+// wave.  TERMS: write every <P_k> (observe's per-term data); otherwise each
+// lane accumulates its share of sum_k c_k <P_k> and the chain writes one
+// partial energy (no per-term wave reductions).
+template <bool TERMS>
+__device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int nitems,
+                                            const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                            const double2 *__restrict__ arena, const double2 *__restrict__ envs,
+                                            double2 *__restrict__ spills, double2 *__restrict__ terms,
+                                            double *__restrict__ parts) {
   const int lane = threadIdx.x & 63;
   // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD
   // a contiguous range of work items so one instance's chains share an L2.
@@ -67,22 +72,29 @@ k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__
   const ChainDev ch = P.chains[it.y];
   const int n = inst.n;
   const double2 *sites = arena + inst.site_off;
-  const double2 *L = envs + inst.env_off;
-  const double2 *Rt = L + (size_t)(n + 1) * GV_ENV;
   double2 *spill = spills + inst.spill_off;
   double2 *tv = terms + inst.term_off;
 
-  // prefix forest: start from L[site0]; suffix forest: from Rt[site0 + 1]
+  // each forest has one root chain starting from the unit environment [1]
+  // (left of site 0 / right of site n-1); every other chain starts from a
+  // spilled node
   Env e;
   if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
-  else if (ch.dir == 0) env_load(L + (size_t)ch.site0 * GV_ENV, lane, e);
-  else env_load(Rt + (size_t)(ch.site0 + 1) * GV_ENV, lane, e);
+  else env_unit(lane, e);
   const bool transposed = ch.dir != 0;
 
+  double acc = 0.0;  // energy mode: this lane's share of sum_k c_k <P_k>
+  StepDev sd = P.steps[ch.step0];
   Frag f;
+  frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
   for (int st = 0; st < ch.nsteps; ++st) {
-    const StepDev sd = P.steps[ch.step0 + st];
-    frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+    // prefetch the next step's site tensor while this step computes
+    StepDev nsd = sd;
+    Frag nf;
+    if (st + 1 < ch.nsteps) {
+      nsd = P.steps[ch.step0 + st + 1];
+      frag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, transposed);
+    }
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
     prod_x(e, f, 1, x1r, x1i);
@@ -110,26 +122,51 @@ k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__
       if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
       // <P> = Tr(E_left R_right) = sum E[b][b'] Rt[b][b'] with the term's
       // suffix-forest environment (or the identity right environment)
+#pragma unroll 2
       for (int t = 0; t < cd.nterm; ++t) {
         const TermEnd te = P.term_list[cd.term0 + t];
         Env r;
         if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
-        else env_load(Rt + (size_t)(sd.site + 1) * GV_ENV, lane, r);
+        else env_unit(lane, r);  // term ends at site n-1
         double sr = 0.0, si = 0.0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           sr += ce.re[v] * r.re[v] - ce.im[v] * r.im[v];
           si += ce.re[v] * r.im[v] + ce.im[v] * r.re[v];
         }
-        sr = wave_sum(sr);
-        si = wave_sum(si);
-        if (lane == 0) tv[te.term] = make_double2(sr, si);
+        if (TERMS) {
+          sr = wave_sum(sr);
+          si = wave_sum(si);
+          if (lane == 0) tv[te.term] = make_double2(sr, si);
+        } else {
+          acc += P.coeffs[2 * te.term] * sr - P.coeffs[2 * te.term + 1] * si;  // Re(c <P>)
+        }
       }
       if (cd.action == -2) cont = ce;
       else if (cd.action >= 0) env_store(spill + (size_t)cd.action * GV_ENV, lane, ce);
     }
     e = cont;
+    sd = nsd;
+    f = nf;
   }
+  if (!TERMS) {
+    acc = wave_sum(acc);
+    if (lane == 0) parts[inst.part_off + it.y] = acc;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+         const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, const double2 *__restrict__ envs,
+         double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  chains_body<true>(items, nitems, insts, probs, arena, envs, spills, terms, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_chains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                const ProbDev *__restrict__ probs, const double2 *__restrict__ arena,
+                const double2 *__restrict__ envs, double2 *__restrict__ spills, double *__restrict__ parts) {
+  chains_body<false>(items, nitems, insts, probs, arena, envs, spills, nullptr, parts);
 }
 
 // E = Re sum_k c_k <P_k> (+ identity words counted as <P> = 1,
@@ -172,6 +209,40 @@ k_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, d
   if (tid == 0) energies[inst.out] = ssum[0] + scomp[0];
 }
 
+// Energy from chain partials: identity words (<P> = 1) plus the partial sums
+// of every chain, compensated and in a fixed order.
+extern "C" __global__ void __launch_bounds__(256)
+k_reduce_energy(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                const double *__restrict__ parts, double *__restrict__ energies) {
+  __shared__ double ssum[256], scomp[256];
+  const int tid = threadIdx.x;
+  const InstDev inst = insts[blockIdx.x];
+  const ProbDev P = probs[inst.prob];
+  double sum = 0.0, comp = 0.0;
+  auto add = [&](double term) {
+    const double t = sum + term;
+    comp += (fabs(sum) >= fabs(term)) ? (sum - t) + term : (term - t) + sum;
+    sum = t;
+  };
+  for (int k = tid; k < P.nterms; k += 256)
+    if (!P.active[k]) add(P.coeffs[2 * k]);
+  for (int c = tid; c < P.nchains; c += 256) add(parts[inst.part_off + c]);
+  ssum[tid] = sum;
+  scomp[tid] = comp;
+  __syncthreads();
+  for (int off = 128; off >= 1; off >>= 1) {
+    if (tid < off) {
+      const double a = ssum[tid], b = ssum[tid + off];
+      const double t = a + b;
+      const double e = (fabs(a) >= fabs(b)) ? (a - t) + b : (b - t) + a;
+      ssum[tid] = t;
+      scomp[tid] += scomp[tid + off] + e;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) energies[inst.out] = ssum[0] + scomp[0];
+}
+
 // ------------------------------------------------------------ host launchers
 namespace gv {
 
@@ -184,6 +255,20 @@ hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev
                      const double2 *arena, const double2 *envs, double2 *spills, double2 *terms) {
   hipLaunchKernelGGL(k_chains, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, arena, envs,
                      spills, terms);
+  return hipGetLastError();
+}
+
+hipError_t kl_chains_energy(int nitems, hipStream_t s, const int2 *items, const InstDev *insts,
+                            const ProbDev *probs, const double2 *arena, const double2 *envs, double2 *spills,
+                            double *parts) {
+  hipLaunchKernelGGL(k_chains_energy, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, arena,
+                     envs, spills, parts);
+  return hipGetLastError();
+}
+
+hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                            const double *parts, double *energies) {
+  hipLaunchKernelGGL(k_reduce_energy, dim3(ninst), dim3(256), 0, s, insts, probs, parts, energies);
   return hipGetLastError();
 }
 

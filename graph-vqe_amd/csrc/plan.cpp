@@ -303,12 +303,15 @@ struct TermPath {
   int k, first, last, c;
 };
 
-// Split choice: term k's left part is [first, c] in the prefix forest, its
-// right part [c+1, last] in the suffix forest.  The number of new nodes for a
-// split c is max(0, c - p0 + 1) + max(0, q - c) where the prefix forest
-// already holds the path up to p0 - 1 and the suffix forest down to q + 1.
-// Greedy: start every term at its middle non-identity site, then re-place
-// each term once at its cheapest split (ties: nearest the middle).
+// Split choice: term k's left path is its word on sites [0, c] in the prefix
+// forest (leading identities included, so every path starts at the unit
+// environment of site 0 and the identity chain is shared), its right path the
+// word on sites [c+1, n-1] in the suffix forest (empty for c = n-1), with
+// first <= c <= last (first/last non-identity sites).  The number of new
+// nodes for a split c is max(0, c - p0 + 1) + max(0, q - c) where the prefix
+// forest already holds the path up to p0 - 1 and the suffix forest down to
+// q + 1.  Greedy: start every term at its middle non-identity site, then
+// re-place each term once at its cheapest split (ties: nearest the middle).
 std::vector<TermPath> choose_splits(const Problem &P) {
   const int n = P.n;
   std::vector<TermPath> paths;
@@ -322,15 +325,15 @@ std::vector<TermPath> choose_splits(const Problem &P) {
   Forest L(n, 0), R(n, 1);
   auto insert = [&](const TermPath &t, int delta) {
     const std::string &w = P.site_words[t.k];
-    L.walk(w, t.first, t.c, delta);
-    if (t.c < t.last) R.walk(w, t.last, t.c + 1, delta);
+    L.walk(w, 0, t.c, delta);
+    if (t.c < n - 1) R.walk(w, n - 1, t.c + 1, delta);
   };
   for (const TermPath &t : paths) insert(t, +1);
   for (TermPath &t : paths) {
     insert(t, -1);
     const std::string &w = P.site_words[t.k];
-    const int p0 = L.first_new(w, t.first, t.last);
-    const int q = R.first_new(w, t.last, t.first);
+    const int p0 = L.first_new(w, 0, t.last);
+    const int q = R.first_new(w, n - 1, t.first);
     const int mid = t.c;
     int best = -1, bc = mid;
     for (int c = t.first; c <= t.last; ++c) {
@@ -437,21 +440,21 @@ void build_chains(Problem &P) {
   for (size_t i = 0; i < paths.size(); ++i) {
     const TermPath &t = paths[i];
     const std::string &w = P.site_words[t.k];
-    if (t.c < t.last) {
-      const int rn = R.walk(w, t.last, t.c + 1, +1);
+    if (t.c < n - 1) {  // c = n-1: the right environment is the unit [1]
+      const int rn = R.walk(w, n - 1, t.c + 1, +1);
       if (R.nodes[rn].store < 0) R.nodes[rn].store = P.nspill++;
       right_leaf[i] = R.nodes[rn].store;
     }
   }
   for (size_t i = 0; i < paths.size(); ++i) {
     const TermPath &t = paths[i];
-    const int ln = L.walk(P.site_words[t.k], t.first, t.c, +1);
+    const int ln = L.walk(P.site_words[t.k], 0, t.c, +1);
     L.nodes[ln].ends.push_back(TermEnd{t.k, right_leaf[i]});
   }
   P.n_nodes = (int)L.nodes.size();
   P.n_right_nodes = (int)R.nodes.size();
-  // Algorithmic node count: 16 (Dl^2 Dr + Dl Dr^2) flops per forest node,
-  // plus the two identity sweeps.
+  // Algorithmic count: 16 (Dl^2 Dr + Dl Dr^2) flops per forest node (the
+  // identity environments are the forests' identity chains).
   const std::vector<int> &bd = P.bonds_pred;
   auto fsite = [&](int s) {
     const double dl = bd[s], dr = bd[s + 1];
@@ -460,10 +463,8 @@ void build_chains(Problem &P) {
   P.flops_nodes = 0;
   for (const TNode &t : L.nodes) P.flops_nodes += fsite(t.site);
   for (const TNode &t : R.nodes) P.flops_nodes += fsite(t.site);
-  for (int s = 0; s < n; ++s) P.flops_nodes += 2.0 * fsite(s);
 
-  const double chi = GV_CHI;
-  P.flops_exec = 2.0 * n * 32.0 * chi * chi * chi;  // identity sweeps, 4 chi^3 complex MACs each
+  P.flops_exec = 0.0;
   std::vector<ChainOut> chains;
   forest_chains(R, n, 0, P.nspill, P.flops_exec, chains);
   int right_gens = 0;

@@ -41,6 +41,17 @@ struct Problem {
   int n_right_nodes = 0;                 // suffix-forest nodes
   std::vector<int> gen_begin;            // chains of generation g: [gen_begin[g], gen_begin[g+1])
   int nspill = 0, ngen = 0, n_nodes = 0;
+  // reverse gradient
+  std::string grad_error;                 // non-empty: the circuit has no bond-doubling form here
+  std::vector<LocalOp> lops;              // per-site local programs
+  std::vector<int> lop_begin;             // n+1
+  std::vector<int> rot_param;             // per rotation id
+  std::vector<double> rot_scale;
+  int ngleaf = 0;
+  std::vector<int> seed_begin;            // nspill+1
+  std::vector<SeedTerm> seed_terms;
+  std::vector<int> site_steps_begin, site_steps;
+  double flops_grad = 0;
   double flops_exec = 0, flops_nodes = 0;
 };
 

@@ -30,6 +30,7 @@ _PROTOS = {
                              ctypes.POINTER(ctypes.c_double), ctypes.c_int],
     'gvmps_problem_destroy': [ctypes.c_void_p],
     'gvmps_problem_get_info': [ctypes.c_void_p, ctypes.POINTER(ProblemInfo)],
+    'gvmps_problem_get_generations': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int],
     'gvmps_batch_create': [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
                            ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int],
     'gvmps_batch_destroy': [ctypes.c_void_p],

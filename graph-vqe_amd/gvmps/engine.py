@@ -73,6 +73,13 @@ class Problem:
         check(lib().gvmps_problem_get_info(self._h, ctypes.byref(inf)))
         return {name: getattr(inf, name) for name, _ in inf._fields_}
 
+    def generations(self):
+        """[(chains, expansion steps)] per chain generation."""
+        g = self.info()['n_generations']
+        out = (ctypes.c_int * (2 * max(g, 1)))()
+        check(lib().gvmps_problem_get_generations(self._h, out, g))
+        return [(out[2 * i], out[2 * i + 1]) for i in range(g)]
+
     def __del__(self):
         h = getattr(self, '_h', None)
         if h is not None and h.value and _lib._lib is not None:

@@ -139,3 +139,12 @@ def test_spsa_40q_config_parity_and_batch_invariance(cuda):
         assert rel_close(e[i], eo), (i, e[i], eo)
     # SPSA pairs differ (the +/- perturbations are not degenerate)
     assert np.all(e[0::2] != e[1::2])
+    # energy-only chains (lane-partial sums) agree with the per-term path
+    small = Batch([p], [4])
+    e_only = small.energies(w['x'][:4]).cpu().numpy()
+    e_terms, tv = small.energies(w['x'][:4], terms=True)
+    e_terms = e_terms.cpu().numpy()
+    assert np.all(np.abs(e_only - e_terms) <= 1e-12 * np.maximum(1.0, np.abs(e_terms)))
+    c = np.asarray(w['coeffs'])
+    tv = tv.cpu().numpy().reshape(4, -1)
+    assert np.all(np.abs((tv * c).sum(axis=1).real - e_terms) <= 1e-10 * np.abs(e_terms))
