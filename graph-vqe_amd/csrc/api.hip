@@ -110,6 +110,18 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     std::vector<int> act(P.nterms, 0);
     for (int k : P.active) act[k] = 1;
     d.active = upload(B->owned, act.data(), act.size());
+    d.nsteps = (int)P.steps.size();
+    d.ngleaf = P.ngleaf;
+    d.nrot = (int)P.rot_param.size();
+    d.lops = upload(B->owned, P.lops.data(), P.lops.size());
+    d.lop_begin = upload(B->owned, P.lop_begin.data(), P.lop_begin.size());
+    d.seed_begin = upload(B->owned, P.seed_begin.data(), P.seed_begin.size());
+    d.seed_terms = upload(B->owned, P.seed_terms.data(), P.seed_terms.size());
+    d.site_steps_begin = upload(B->owned, P.site_steps_begin.data(), P.site_steps_begin.size());
+    d.site_steps = upload(B->owned, P.site_steps.data(), P.site_steps.size());
+    d.param_rot_begin = upload(B->owned, P.param_rot_begin.data(), P.param_rot_begin.size());
+    d.param_rots = upload(B->owned, P.param_rots.data(), P.param_rots.size());
+    d.rot_scale = upload(B->owned, P.rot_scale.data(), P.rot_scale.size());
   }
   B->d_probs = upload(B->owned, pd.data(), pd.size());
   // Z masks (site order) for the shot estimator
@@ -130,7 +142,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   }
   B->d_masks = upload(B->owned, mask_ptrs.data(), mask_ptrs.size());
   // instances
-  long long site = 0, env = 0, spill = 0, term = 0, param = 0, part = 0;
+  long long site = 0, env = 0, spill = 0, term = 0, param = 0, part = 0, rot = 0, step = 0, gleaf = 0;
   for (int j = 0; j < n_problems; ++j) {
     const Problem &P = *problems[j]->p;
     if (counts[j] < 0) throw std::runtime_error("negative instance count");
@@ -145,7 +157,15 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       in.param_off = param;
       in.part_off = part;
       in.out = (int)B->inst.size();
+      in.rot_off = rot;
+      in.step_off = step;
+      in.gleaf_off = gleaf;
       B->inst.push_back(in);
+      rot += (long long)P.rot_param.size();
+      step += (long long)P.steps.size();
+      gleaf += P.ngleaf;
+      B->max_n = std::max(B->max_n, P.n);
+      B->max_params = std::max(B->max_params, P.nparams);
       site += (long long)P.n * GV_SLOT;
       env += 2LL * (P.n + 1) * GV_ENV;
       spill += (long long)P.nspill * GV_ENV;
@@ -158,6 +178,11 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   if (B->inst.empty()) throw std::runtime_error("empty batch");
   B->n_terms_total = term;
   B->n_params_total = param;
+  B->n_rot_total = rot;
+  B->n_steps_total = step;
+  B->n_gleaf_total = gleaf;
+  B->n_site_total = site;
+  B->n_spill_total = spill;
   B->d_inst = upload(B->owned, B->inst.data(), B->inst.size());
   B->arena = upload<double2>(B->owned, nullptr, site);
   B->envs = upload<double2>(B->owned, nullptr, env);
@@ -265,6 +290,57 @@ unsigned long long *launch_sample(Batch &B, const double *params, int shots, con
   return packed;
 }
 
+// Reverse gradient over the bond-doubling state.  Phases (timing slots):
+// 0 local build, 1 forward chains, 2 energy reduce, 3 backward chains,
+// 4 site-adjoint reduce, 5 local backward + parameter sums.
+void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
+  for (const Problem *P : B.probs)
+    if (!P->grad_error.empty()) throw std::runtime_error(P->grad_error);
+  if (!params && B.n_params_total > 0) throw std::runtime_error("null parameter array");
+  const int ninst = (int)B.inst.size();
+  if (!B.grad_ready) {
+    B.garena = upload<double2>(B.owned, nullptr, B.n_site_total);
+    B.abar = upload<double2>(B.owned, nullptr, B.n_site_total);
+    B.rotin = upload<double2>(B.owned, nullptr, (size_t)B.n_rot_total * GV_SLOT);
+    B.rgrad = upload<double>(B.owned, nullptr, B.n_rot_total);
+    B.estore = upload<double2>(B.owned, nullptr, (size_t)B.n_steps_total * GV_ENV);
+    B.aslot = upload<double2>(B.owned, nullptr, (size_t)B.n_steps_total * GV_SLOT);
+    B.gleafs = upload<double2>(B.owned, nullptr, (size_t)B.n_gleaf_total * GV_ENV);
+    B.adjspill = upload<double2>(B.owned, nullptr, B.n_spill_total);
+    B.grads = upload<double>(B.owned, nullptr, B.n_params_total);
+    B.grad_ready = true;
+  }
+  B.mark(0, stream);
+  hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.garena, B.rotin),
+            "k_build_local launch");
+  B.mark(1, stream);
+  for (int g = 0; g < B.max_gen; ++g) {
+    const int ni = B.n_items[g];
+    if (ni == 0) continue;
+    hip_check(kl_chains_gfwd(ni, stream, B.items[g], B.d_inst, B.d_probs, B.garena, B.spills, B.parts, B.estore,
+                             B.gleafs),
+              "k_chains_gfwd launch");
+  }
+  B.mark(2, stream);
+  hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies), "k_reduce launch");
+  B.mark(3, stream);
+  for (int g = B.max_gen - 1; g >= 0; --g) {
+    const int ni = B.n_items[g];
+    if (ni == 0) continue;
+    hip_check(kl_chains_bwd(ni, stream, B.items[g], B.d_inst, B.d_probs, B.garena, B.spills, B.adjspill, B.estore,
+                            B.gleafs, B.aslot),
+              "k_chains_bwd launch");
+  }
+  B.mark(4, stream);
+  hip_check(kl_site_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_site_reduce launch");
+  B.mark(5, stream);
+  hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad),
+            "k_site_bwd launch");
+  hip_check(kl_param_grad(ninst, B.max_params, stream, B.d_inst, B.d_probs, B.rgrad, B.grads),
+            "k_param_grad launch");
+  B.mark(6, stream);
+}
+
 }  // namespace gv
 
 // ------------------------------------------------------------------ C ABI
@@ -341,6 +417,9 @@ int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
     info->flops_energy_executed = P.flops_exec;
     info->flops_energy_nodes = P.flops_nodes;
     info->flops_svd_nominal = P.flops_svd;
+    info->flops_grad_executed = P.flops_grad;
+    info->n_rotations = (int)P.rot_param.size();
+    info->grad_ok = P.grad_error.empty() ? 1 : 0;
   })
 }
 
@@ -446,6 +525,25 @@ int gvmps_batch_prep_stats(gvmps_batch *b, int *sweeps_host) {
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     hip_check(hipMemcpy(sweeps_host, B.stats, sizeof(int) * B.inst.size() * GV_STATS, hipMemcpyDeviceToHost),
               "read stats");
+  })
+}
+
+int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *energies_dev, double *grads_dev,
+                           void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hipStream_t s = (hipStream_t)stream;
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    launch_value_grad(B, params_dev, s);
+    if (energies_dev)
+      hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * B.inst.size(),
+                               hipMemcpyDeviceToDevice, s),
+                "copy energies");
+    if (grads_dev && B.n_params_total > 0)
+      hip_check(hipMemcpyAsync(grads_dev, B.grads, sizeof(double) * B.n_params_total, hipMemcpyDeviceToDevice, s),
+                "copy gradients");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    B.collect();
   })
 }
 

@@ -36,6 +36,13 @@ struct Batch {
   std::vector<int> n_items;
   int max_gen = 0;
   long long n_terms_total = 0, n_params_total = 0;
+  int max_n = 0, max_params = 0;
+  // reverse gradient (allocated on first use)
+  long long n_rot_total = 0, n_steps_total = 0, n_gleaf_total = 0, n_site_total = 0, n_spill_total = 0;
+  bool grad_ready = false;
+  double2 *garena = nullptr, *rotin = nullptr, *estore = nullptr, *gleafs = nullptr, *adjspill = nullptr,
+          *aslot = nullptr, *abar = nullptr;
+  double *rgrad = nullptr, *grads = nullptr;
   // optional per-phase HIP-event timing (prep, envs, chains, reduce, sample, grad)
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -67,6 +74,21 @@ hipError_t kl_chains_energy(int nitems, hipStream_t s, const int2 *items, const 
                             double *parts);
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                             const double *parts, double *energies);
+void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
+// kernel launchers (kernels_grad.hip)
+hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                          const double *params, double2 *garena, double2 *rotin);
+hipError_t kl_chains_gfwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                          const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs);
+hipError_t kl_chains_bwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                         const double2 *garena, const double2 *spills, double2 *adjspill, const double2 *estore,
+                         const double2 *gleafs, double2 *aslot);
+hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                          const double2 *aslot, double2 *abar);
+hipError_t kl_site_bwd(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad);
+hipError_t kl_param_grad(int ninst, int max_params, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                         const double *rgrad, double *grads);
 // kernel launchers (kernels_sample.hip)
 hipError_t kl_sample(int ninst, int shots, int stride, hipStream_t s, const InstDev *insts, const double2 *arena,
                      const double2 *envs, const unsigned *seeds, unsigned char *bits, unsigned long long *packed);

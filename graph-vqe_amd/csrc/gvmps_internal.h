@@ -138,8 +138,9 @@ struct ProbDev {
   const SeedTerm* seed_terms;
   const int* site_steps_begin;  // n+1 offsets
   const int* site_steps;   // steps at each site (fixed summation order)
-  const int* rot_param;    // per rotation: parameter index (-1 constant)
-  const double* rot_scale;
+  const int* param_rot_begin;   // nparams+1 offsets: rotations driven by each parameter
+  const int* param_rots;
+  const double* rot_scale;      // per rotation: d angle / d parameter
 };
 
 // One MPS instance of a batch.
@@ -154,6 +155,10 @@ struct InstDev {
   long long part_off;  // chain partial energies (one double per chain)
   int out;             // output index (energy slot)
   int pad;
+  // reverse gradient arenas (in slots of their own kind)
+  long long rot_off;   // rotation inputs (8 KiB) and rotation gradients (one double)
+  long long step_off;  // step input environments (4 KiB) and step site adjoints (8 KiB)
+  long long gleaf_off; // kept prefix leaf environments (4 KiB)
 };
 
 // Status codes written by kernels (per instance).

@@ -1,0 +1,538 @@
+// Reverse-gradient kernels (gfx950): value and dE/dx of every instance.
+//
+// The reverse engine of the reference (test/tools/reverse_mps.py:28-99,
+// 141-184) differentiates an MPS whose CNOTs are applied as exact bond
+// doublings, so every site tensor is a function of its own gates only.  The
+// pass here:
+//   k_build_local   one wave per (instance, site): the site's local program
+//                   (rotations, fixed gates, CNOT halves) from [1, 0]; keeps
+//                   each rotation's input tensor for the backward pass
+//   k_chains_gfwd   the energy forests of kernels_energy.hip, storing every
+//                   step's input environment and the prefix leaves whose
+//                   terms end in a suffix-forest slot
+//   k_chains_bwd    one wave per (instance, chain), generations in reverse:
+//                   adjoint environments ride in registers down the chain,
+//                   spilled children hand theirs over through HBM; every
+//                   step writes its site-tensor adjoint to a slot of its own
+//   k_site_reduce   per (instance, site): sum of its steps' adjoints, in a
+//                   fixed order (deterministic)
+//   k_site_bwd      one wave per (instance, site): the local program in
+//                   reverse -> d E / d angle of every rotation
+//   k_param_grad    per (instance, parameter): sum over its rotations
+//
+// Adjoint convention: for a real loss L and complex z, zbar = dL/dRe z +
+// i dL/dIm z, so dL = Re sum conj(zbar) dz.  The loss is
+// sum_k Re(c_k) Re<P_k> (real coefficients, reverse_mps.py:78-81).
+#include "device_common.h"
+
+using namespace gv;
+
+namespace {
+
+// U(theta) = cos(theta/2) I - i sin(theta/2) sigma_axis and dU/dtheta, in the
+// branch-free form of the state-preparation kernel.
+__device__ __forceinline__ void rot_pair(int axis, double theta, double2 U[4], double2 dU[4]) {
+  double sn, cs;
+  sincos(0.5 * theta, &sn, &cs);
+  const double fx = axis == 0 ? 1.0 : 0.0, fy = axis == 1 ? 1.0 : 0.0, fz = axis == 2 ? 1.0 : 0.0;
+  U[0] = make_double2(cs, -sn * fz);
+  U[1] = make_double2(-sn * fy, -sn * fx);
+  U[2] = make_double2(sn * fy, -sn * fx);
+  U[3] = make_double2(cs, sn * fz);
+  const double h = 0.5;
+  dU[0] = make_double2(-h * sn, -h * cs * fz);
+  dU[1] = make_double2(-h * cs * fy, -h * cs * fx);
+  dU[2] = make_double2(h * cs * fy, -h * cs * fx);
+  dU[3] = make_double2(-h * sn, h * cs * fz);
+}
+
+__device__ __forceinline__ void local_matrix(const LocalOp &op, const double *x, const double *fixes,
+                                             double2 U[4], double2 dU[4]) {
+  if (op.kind == LO_ROT) {
+    const double theta = (op.param >= 0 ? op.scale * x[op.param] : 0.0) + op.offset;
+    rot_pair(op.arg, theta, U, dU);
+  } else {
+    const double *F = fixes + 8 * (op.arg < 0 ? 0 : op.arg);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      U[e] = make_double2(F[2 * e], F[2 * e + 1]);
+      dU[e] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// Slot element e = (l*2 + i)*16 + r.
+__device__ __forceinline__ int sidx(int l, int i, int r) { return (l * 2 + i) * 16 + r; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ build
+// grid (max sites, instances), one wave per block.
+extern "C" __global__ void __launch_bounds__(64)
+k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+              const double *__restrict__ params, double2 *__restrict__ garena, double2 *__restrict__ rotin) {
+  __shared__ double2 buf[2][GV_SLOT];
+  const int lane = threadIdx.x;
+  const InstDev inst = insts[blockIdx.y];
+  const int s = blockIdx.x;
+  if (s >= inst.n) return;
+  const ProbDev &P = probs[inst.prob];
+  const double *x = params + inst.param_off;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int e = lane + 64 * k;
+    buf[0][e] = make_double2(e == 0 ? 1.0 : 0.0, 0.0);  // |0> of a (1,2,1) site
+  }
+  __syncthreads();
+  int cur = 0;
+  const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1];
+  for (int o = o0; o < o1; ++o) {
+    const LocalOp op = P.lops[o];
+    double2 U[4], dU[4];
+    local_matrix(op, x, P.fixes, U, dU);
+    double2 *keep = op.kind == LO_ROT ? rotin + (inst.rot_off + op.rot) * GV_SLOT : nullptr;
+    const double2 *in = buf[cur];
+    double2 *out = buf[cur ^ 1];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = lane + 64 * k;
+      const int l = e >> 5, i = (e >> 4) & 1, r = e & 15;
+      if (keep) keep[e] = in[e];
+      double2 v;
+      switch (op.kind) {
+        case LO_CTRL_R:  // A'[l,i,2r+s] = A[l,i,r] d(i,s)
+          v = ((r & 1) == i) ? in[sidx(l, i, r >> 1)] : make_double2(0.0, 0.0);
+          break;
+        case LO_CTRL_L:  // A'[2l+s,i,r] = A[l,i,r] d(i,s)
+          v = ((l & 1) == i) ? in[sidx(l >> 1, i, r)] : make_double2(0.0, 0.0);
+          break;
+        case LO_TARG_R:  // A'[l,i,2r+s] = A[l,i^s,r]
+          v = in[sidx(l, i ^ (r & 1), r >> 1)];
+          break;
+        case LO_TARG_L:  // A'[2l+s,i,r] = A[l,i^s,r]
+          v = in[sidx(l >> 1, i ^ (l & 1), r)];
+          break;
+        default:  // 2x2 gate on the physical index
+          v = cadd(cmul(i ? U[2] : U[0], in[sidx(l, 0, r)]), cmul(i ? U[3] : U[1], in[sidx(l, 1, r)]));
+      }
+      out[e] = v;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  double2 *dst = garena + inst.site_off + (size_t)s * GV_SLOT;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dst[lane + 64 * k] = buf[cur][lane + 64 * k];
+}
+
+// --------------------------------------------------------------- forward
+// Energy chains on the bond-doubling state, keeping what the backward pass
+// reads: the input environment of every step and the prefix leaves that end
+// terms against a suffix-forest slot.
+extern "C" __global__ void __launch_bounds__(256)
+k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+              const ProbDev *__restrict__ probs, const double2 *__restrict__ garena, double2 *__restrict__ spills,
+              double *__restrict__ parts, double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
+  const int lane = threadIdx.x & 63;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+  const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
+  const int w = logical * 4 + (threadIdx.x >> 6);
+  if (w >= nitems) return;
+  const int2 it = items[w];
+  const InstDev inst = insts[it.x];
+  const ProbDev &P = probs[inst.prob];
+  const ChainDev ch = P.chains[it.y];
+  const double2 *sites = garena + inst.site_off;
+  double2 *spill = spills + inst.spill_off;
+  double2 *es = estore + inst.step_off * GV_ENV;
+  double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
+  Env e;
+  if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
+  else env_unit(lane, e);
+  const bool transposed = ch.dir != 0;
+  double acc = 0.0;
+  for (int st = 0; st < ch.nsteps; ++st) {
+    const int step = ch.step0 + st;
+    const StepDev sd = P.steps[step];
+    Frag f;
+    frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+    env_store(es + (size_t)step * GV_ENV, lane, e);
+    d4 x0r, x0i, x1r, x1i;
+    prod_x(e, f, 0, x0r, x0i);
+    prod_x(e, f, 1, x1r, x1i);
+    d4 y00r, y00i, y11r, y11i, y01r, y01i, y10r, y10i;
+    if (sd.need & 1) {
+      prod_y(x0r, x0i, f, 0, y00r, y00i);
+      prod_y(x1r, x1i, f, 1, y11r, y11i);
+    }
+    if (sd.need & 2) {
+      prod_y(x1r, x1i, f, 0, y01r, y01i);
+      prod_y(x0r, x0i, f, 1, y10r, y10i);
+    }
+    Env cont;
+    for (int c = 0; c < sd.nchild; ++c) {
+      const ChildDev cd = P.children[sd.child0 + c];
+      Env ce;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if (cd.kind == CK_I) { ce.re[v] = y00r[v] + y11r[v]; ce.im[v] = y00i[v] + y11i[v]; }
+        else if (cd.kind == CK_Z) { ce.re[v] = y00r[v] - y11r[v]; ce.im[v] = y00i[v] - y11i[v]; }
+        else if (cd.kind == CK_X) { ce.re[v] = y01r[v] + y10r[v]; ce.im[v] = y01i[v] + y10i[v]; }
+        else { ce.re[v] = y01i[v] - y10i[v]; ce.im[v] = -y01r[v] + y10r[v]; }
+      }
+      if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
+      if (cd.gleaf >= 0) env_store(gl + (size_t)cd.gleaf * GV_ENV, lane, ce);
+      for (int t = 0; t < cd.nterm; ++t) {
+        const TermEnd te = P.term_list[cd.term0 + t];
+        Env r;
+        if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+        else env_unit(lane, r);
+        double sr = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sr += ce.re[v] * r.re[v] - ce.im[v] * r.im[v];
+        acc += P.coeffs[2 * te.term] * sr;  // Re(c_k) Re<P_k>
+      }
+      if (cd.action == -2) cont = ce;
+      else if (cd.action >= 0) env_store(spill + (size_t)cd.action * GV_ENV, lane, ce);
+    }
+    e = cont;
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) parts[inst.part_off + it.y] = acc;
+}
+
+// -------------------------------------------------------------- backward
+namespace {
+
+// D += A B for complex 16x16 operands given as MFMA registers: `a` in E layout
+// (A operand), `b` in D layout (B operand).  ca / cb conjugate the operand.
+__device__ __forceinline__ void cmm(const double *ar, const double *ai, const double *br, const double *bi,
+                                    bool ca, bool cb, d4 &dr, d4 &di) {
+  const double sa = ca ? -1.0 : 1.0, sb = cb ? -1.0 : 1.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const double ait = sa * ai[t], bit = sb * bi[t];
+    dr = mfma(ar[t], br[t], dr);
+    dr = mfma(-ait, bit, dr);
+    di = mfma(ar[t], bit, di);
+    di = mfma(ait, br[t], di);
+  }
+}
+
+__device__ __forceinline__ void env_zero(Env &e) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) e.re[t] = e.im[t] = 0.0;
+}
+
+// e += w * conj(q) elementwise (same layout)
+__device__ __forceinline__ void add_conj(Env &e, const Env &q, double w) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    e.re[t] += w * q.re[t];
+    e.im[t] -= w * q.im[t];
+  }
+}
+
+}  // namespace
+
+// One generation of chains, backward.  For a step with input environment E
+// and site matrices M_i (A_i, or A_i^T in the suffix forest) the forward
+// computed Y_ji = M_j^H E M_i and children I = Y00+Y11, Z = Y00-Y11,
+// X = Y01+Y10, Y = -i Y01 + i Y10 (analytic_batch.cuh:11-14).  With child
+// adjoints Cbar: Ybar00 = I+Z, Ybar11 = I-Z, Ybar01 = X + iY, Ybar10 = X - iY
+// (Hermitian pairs, Ybar10 = Ybar01^H), and with G_i = sum_j M_j Ybar_ji
+//   Ebar   = sum_i G_i M_i^H
+//   Mbar_i = 2 E G_i            (E Hermitian)
+extern "C" __global__ void __launch_bounds__(256)
+k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+             const ProbDev *__restrict__ probs, const double2 *__restrict__ garena,
+             const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
+             const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
+  __shared__ double2 tr[4][16][17];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+  const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
+  const int w = logical * 4 + wv;
+  if (w >= nitems) return;
+  const int2 it = items[w];
+  const InstDev inst = insts[it.x];
+  const ProbDev &P = probs[inst.prob];
+  const ChainDev ch = P.chains[it.y];
+  const double2 *sites = garena + inst.site_off;
+  const double2 *spill = spills + inst.spill_off;
+  double2 *aspill = adjspill + inst.spill_off;
+  const double2 *es = estore + inst.step_off * GV_ENV;
+  const double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
+  double2 *as = aslot + inst.step_off * GV_SLOT;
+  const bool transposed = ch.dir != 0;
+  const int a16 = lane & 15, g4 = lane >> 4;
+
+  Env eb;  // adjoint of the environment continuing the chain
+  env_zero(eb);
+  for (int st = ch.nsteps - 1; st >= 0; --st) {
+    const int step = ch.step0 + st;
+    const StepDev sd = P.steps[step];
+    Env E;
+    env_load(es + (size_t)step * GV_ENV, lane, E);
+    Frag fa;  // E layout of M_i (A operand form)
+    frag_load(sites + (size_t)sd.site * GV_SLOT, lane, fa, !transposed);
+    Env y00, y11, y01, y10;
+    env_zero(y00);
+    env_zero(y11);
+    env_zero(y01);
+    env_zero(y10);
+    for (int c = 0; c < sd.nchild; ++c) {
+      const ChildDev cd = P.children[sd.child0 + c];
+      Env cb;
+      if (cd.action == -2) cb = eb;
+      else if (cd.action >= 0) env_load(aspill + (size_t)cd.action * GV_ENV, lane, cb);
+      else env_zero(cb);
+      // term ends (prefix forest): Cbar += Re(c_k) conj(Rt_k)
+      for (int t = 0; t < cd.nterm; ++t) {
+        const TermEnd te = P.term_list[cd.term0 + t];
+        const double ck = P.coeffs[2 * te.term];
+        if (te.rslot >= 0) {
+          Env r;
+          env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+          add_conj(cb, r, ck);
+        } else if (lane == 0) {
+          cb.re[0] += ck;  // unit right environment
+        }
+      }
+      // suffix-forest slot read by prefix term ends: Cbar += Re(c_k) conj(E_left(k))
+      if (cd.store >= 0) {
+        for (int q = P.seed_begin[cd.store]; q < P.seed_begin[cd.store + 1]; ++q) {
+          const SeedTerm sdt = P.seed_terms[q];
+          Env le;
+          env_load(gl + (size_t)sdt.gleaf * GV_ENV, lane, le);
+          add_conj(cb, le, P.coeffs[2 * sdt.term]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double re = cb.re[t], im = cb.im[t];
+        if (cd.kind == CK_I) { y00.re[t] += re; y00.im[t] += im; y11.re[t] += re; y11.im[t] += im; }
+        else if (cd.kind == CK_Z) { y00.re[t] += re; y00.im[t] += im; y11.re[t] -= re; y11.im[t] -= im; }
+        else if (cd.kind == CK_X) { y01.re[t] += re; y01.im[t] += im; y10.re[t] += re; y10.im[t] += im; }
+        else {  // Ybar01 += i Cbar, Ybar10 -= i Cbar
+          y01.re[t] -= im; y01.im[t] += re; y10.re[t] += im; y10.im[t] -= re;
+        }
+      }
+    }
+    // G_i in E layout, computed as G_i^T = sum_j Ybar_ji^T M_j^T with
+    // Ybar00^T = conj(Ybar00), Ybar10^T = conj(Ybar01), ...
+    d4 g0r = {0, 0, 0, 0}, g0i = {0, 0, 0, 0}, g1r = {0, 0, 0, 0}, g1i = {0, 0, 0, 0};
+    if (sd.need & 1) {
+      cmm(y00.re, y00.im, fa.re[0], fa.im[0], true, false, g0r, g0i);
+      cmm(y11.re, y11.im, fa.re[1], fa.im[1], true, false, g1r, g1i);
+    }
+    if (sd.need & 2) {
+      cmm(y01.re, y01.im, fa.re[1], fa.im[1], true, false, g0r, g0i);
+      cmm(y10.re, y10.im, fa.re[0], fa.im[0], true, false, g1r, g1i);
+    }
+    // Ebar^T = sum_i (G_i M_i^H)^T -> conj gives Ebar in E layout (Hermitian)
+    d4 er = {0, 0, 0, 0}, ei = {0, 0, 0, 0};
+    {
+      double t0r[4], t0i[4], t1r[4], t1i[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        t0r[t] = g0r[t]; t0i[t] = g0i[t]; t1r[t] = g1r[t]; t1i[t] = g1i[t];
+      }
+      cmm(t0r, t0i, fa.re[0], fa.im[0], false, true, er, ei);
+      cmm(t1r, t1i, fa.re[1], fa.im[1], false, true, er, ei);
+    }
+    // Mbar_i = 2 E G_i: G_i to D layout through LDS
+    double2 *slot = as + (size_t)step * GV_SLOT;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const d4 &gr = i ? g1r : g0r;
+      const d4 &gi = i ? g1i : g0i;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) tr[wv][a16][kdx(g4, t)] = make_double2(gr[t], gi[t]);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double dr[4], di[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double2 q = tr[wv][kdx(g4, v)][a16];
+        dr[v] = 2.0 * q.x;
+        di[v] = 2.0 * q.y;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      d4 mr = {0, 0, 0, 0}, mi = {0, 0, 0, 0};
+      cmm(E.re, E.im, dr, di, false, false, mr, mi);
+      // D layout: lane (r + 16 g), register v = Mbar_i[kdx(g, v)][r]
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = kdx(g4, v);
+        const int idx = transposed ? sidx(a16, i, row) : sidx(row, i, a16);
+        slot[idx] = make_double2(mr[v], mi[v]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      eb.re[t] = er[t];
+      eb.im[t] = -ei[t];
+    }
+  }
+  if (ch.src >= 0) env_store(aspill + (size_t)ch.src * GV_ENV, lane, eb);
+}
+
+// ------------------------------------------------------- site adjoints
+// grid (max sites, instances), 256 threads: thread t owns doubles 4t..4t+3
+// of the 16x2x16 complex slot.
+extern "C" __global__ void __launch_bounds__(256)
+k_site_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+              const double2 *__restrict__ aslot, double2 *__restrict__ abar) {
+  const InstDev inst = insts[blockIdx.y];
+  const int s = blockIdx.x;
+  if (s >= inst.n) return;
+  const ProbDev &P = probs[inst.prob];
+  const int t = threadIdx.x;
+  const double2 *as = aslot + inst.step_off * GV_SLOT;
+  double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
+  for (int q = P.site_steps_begin[s]; q < P.site_steps_begin[s + 1]; ++q) {
+    const double2 *src = as + (size_t)P.site_steps[q] * GV_SLOT;
+    const double2 a = src[2 * t], c = src[2 * t + 1];
+    s0 = cadd(s0, a);
+    s1 = cadd(s1, c);
+  }
+  double2 *dst = abar + inst.site_off + (size_t)s * GV_SLOT;
+  dst[2 * t] = s0;
+  dst[2 * t + 1] = s1;
+}
+
+// Local program in reverse.  Rotation k: T_{k+1} = U T_k, so
+// dE/dtheta = Re sum conj(Tbar_{k+1}) (dU/dtheta T_k) and Tbar_k = U^H Tbar_{k+1};
+// CNOT halves pull the adjoint back through their index maps.
+extern "C" __global__ void __launch_bounds__(64)
+k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
+           const double2 *__restrict__ abar, const double2 *__restrict__ rotin, double *__restrict__ rgrad) {
+  __shared__ double2 buf[2][GV_SLOT];
+  const int lane = threadIdx.x;
+  const InstDev inst = insts[blockIdx.y];
+  const int s = blockIdx.x;
+  if (s >= inst.n) return;
+  const ProbDev &P = probs[inst.prob];
+  const double *x = params + inst.param_off;
+  const double2 *src = abar + inst.site_off + (size_t)s * GV_SLOT;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) buf[0][lane + 64 * k] = src[lane + 64 * k];
+  __syncthreads();
+  int cur = 0;
+  const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1];
+  for (int o = o1 - 1; o >= o0; --o) {
+    const LocalOp op = P.lops[o];
+    double2 U[4], dU[4];
+    local_matrix(op, x, P.fixes, U, dU);
+    const double2 *in = buf[cur];
+    double2 *out = buf[cur ^ 1];
+    if (op.kind == LO_ROT) {
+      const double2 *T = rotin + (inst.rot_off + op.rot) * GV_SLOT;
+      double part = 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int e = lane + 64 * k;
+        const int l = e >> 5, i = (e >> 4) & 1, r = e & 15;
+        const double2 d = cadd(cmul(i ? dU[2] : dU[0], T[sidx(l, 0, r)]), cmul(i ? dU[3] : dU[1], T[sidx(l, 1, r)]));
+        const double2 tb = in[e];
+        part += tb.x * d.x + tb.y * d.y;  // Re(conj(tb) d)
+      }
+      part = wave_sum(part);
+      if (lane == 0) rgrad[inst.rot_off + op.rot] = part;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = lane + 64 * k;
+      const int l = e >> 5, i = (e >> 4) & 1, r = e & 15;
+      double2 v = make_double2(0.0, 0.0);
+      switch (op.kind) {
+        case LO_CTRL_R:
+          if (r < 8) v = in[sidx(l, i, 2 * r + i)];
+          break;
+        case LO_CTRL_L:
+          if (l < 8) v = in[sidx(2 * l + i, i, r)];
+          break;
+        case LO_TARG_R:
+          if (r < 8) v = cadd(in[sidx(l, i, 2 * r)], in[sidx(l, i ^ 1, 2 * r + 1)]);
+          break;
+        case LO_TARG_L:
+          if (l < 8) v = cadd(in[sidx(2 * l, i, r)], in[sidx(2 * l + 1, i ^ 1, r)]);
+          break;
+        default:  // Tbar_k[l,j,r] = sum_i conj(U[i][j]) Tbar[l,i,r]
+          v = cadd(cmul(cconj(i ? U[1] : U[0]), in[sidx(l, 0, r)]), cmul(cconj(i ? U[3] : U[2]), in[sidx(l, 1, r)]));
+      }
+      out[e] = v;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+// dE/dx_p = sum over p's rotations of scale * dE/dangle, in rotation order.
+extern "C" __global__ void __launch_bounds__(256)
+k_param_grad(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ rgrad,
+             double *__restrict__ grads) {
+  const InstDev inst = insts[blockIdx.y];
+  const ProbDev &P = probs[inst.prob];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= P.nparams) return;
+  double g = 0.0;
+  for (int q = P.param_rot_begin[p]; q < P.param_rot_begin[p + 1]; ++q) {
+    const int r = P.param_rots[q];
+    g += P.rot_scale[r] * rgrad[inst.rot_off + r];
+  }
+  grads[inst.param_off + p] = g;
+}
+
+// ------------------------------------------------------------ host launchers
+namespace gv {
+
+hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                          const double *params, double2 *garena, double2 *rotin) {
+  hipLaunchKernelGGL(k_build_local, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, garena, rotin);
+  return hipGetLastError();
+}
+
+hipError_t kl_chains_gfwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                          const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs) {
+  hipLaunchKernelGGL(k_chains_gfwd, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, garena,
+                     spills, parts, estore, gleafs);
+  return hipGetLastError();
+}
+
+hipError_t kl_chains_bwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                         const double2 *garena, const double2 *spills, double2 *adjspill, const double2 *estore,
+                         const double2 *gleafs, double2 *aslot) {
+  hipLaunchKernelGGL(k_chains_bwd, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, garena,
+                     spills, adjspill, estore, gleafs, aslot);
+  return hipGetLastError();
+}
+
+hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                          const double2 *aslot, double2 *abar) {
+  hipLaunchKernelGGL(k_site_reduce, dim3(max_n, ninst), dim3(256), 0, s, insts, probs, aslot, abar);
+  return hipGetLastError();
+}
+
+hipError_t kl_site_bwd(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad) {
+  hipLaunchKernelGGL(k_site_bwd, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, abar, rotin, rgrad);
+  return hipGetLastError();
+}
+
+hipError_t kl_param_grad(int ninst, int max_params, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                         const double *rgrad, double *grads) {
+  if (max_params <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_param_grad, dim3((max_params + 255) / 256, ninst), dim3(256), 0, s, insts, probs, rgrad,
+                     grads);
+  return hipGetLastError();
+}
+
+}  // namespace gv

@@ -511,6 +511,165 @@ void build_chains(Problem &P) {
   P.ngen = (int)P.gen_begin.size() - 1;
 }
 
+// Per-site programs of the bond-doubling state used by the reverse gradient
+// (test/tools/reverse_mps.py:39-57, 152-184): every site tensor is built from
+// [1, 0] by its own single-qubit gates and the CNOT halves that touch it, so
+// sites are independent.  A neighbouring CNOT doubles the shared bond exactly;
+// anything else (SWAP, distant CNOT, a bond beyond the 16-wide slots) leaves
+// the problem without a reverse gradient (grad_error says why).
+void build_local_programs(Problem &P, const gvmps_op *ops, int n_ops) {
+  const int n = P.n;
+  std::vector<std::vector<LocalOp>> prog(n);
+  std::vector<int> bond(n + 1, 1);
+  P.rot_param.clear();
+  P.rot_scale.clear();
+  P.grad_error.clear();
+  auto fail = [&](const std::string &why) {
+    if (P.grad_error.empty()) P.grad_error = why;
+  };
+  for (int k = 0; k < n_ops && P.grad_error.empty(); ++k) {
+    const gvmps_op &o = ops[k];
+    if (o.kind == GVMPS_RX || o.kind == GVMPS_RY || o.kind == GVMPS_RZ) {
+      const int s = n - 1 - o.q0;
+      LocalOp lo{};
+      lo.kind = LO_ROT;
+      lo.arg = o.kind - GVMPS_RX;
+      lo.param = o.param;
+      lo.scale = o.param >= 0 ? o.scale : 0.0;
+      lo.offset = o.offset;
+      lo.rot = (int)P.rot_param.size();
+      P.rot_param.push_back(o.param);
+      P.rot_scale.push_back(lo.scale);
+      prog[s].push_back(lo);
+    } else if (o.kind == GVMPS_CX) {
+      const int sc = n - 1 - o.q0, st = n - 1 - o.q1;
+      if (std::abs(sc - st) != 1) {
+        fail("reverse gradient: CX(" + std::to_string(o.q0) + "," + std::to_string(o.q1) +
+             ") is not between neighbouring qubits (bond doubling needs adjacent CNOTs)");
+        break;
+      }
+      const int lo_s = std::min(sc, st);
+      if (2 * bond[lo_s + 1] > GV_CHI) {
+        fail("reverse gradient: CNOT bond doubling exceeds the 16-wide bond slots");
+        break;
+      }
+      bond[lo_s + 1] *= 2;
+      LocalOp a{}, b{};
+      a.param = b.param = -1;
+      a.rot = b.rot = -1;
+      if (sc == lo_s) {  // control on the left site
+        a.kind = LO_CTRL_R;
+        b.kind = LO_TARG_L;
+      } else {
+        a.kind = LO_TARG_R;
+        b.kind = LO_CTRL_L;
+      }
+      prog[lo_s].push_back(a);
+      prog[lo_s + 1].push_back(b);
+    } else if (o.kind == GVMPS_SWAP) {
+      fail("reverse gradient: SWAP has no bond-doubling form");
+    } else {
+      LocalOp lo{};
+      lo.kind = LO_FIX;
+      lo.arg = -1;
+      const M2 U = fixed_matrix(o.kind);
+      // fix table shared with the state-preparation program
+      const int count = (int)P.fixes.size() / 8;
+      for (int g = 0; g < count && lo.arg < 0; ++g) {
+        bool same = true;
+        for (int e = 0; e < 4 && same; ++e)
+          same = P.fixes[g * 8 + 2 * e] == U[e].real() && P.fixes[g * 8 + 2 * e + 1] == U[e].imag();
+        if (same) lo.arg = g;
+      }
+      if (lo.arg < 0) throw std::runtime_error("internal: fixed gate missing from the table");
+      lo.param = lo.rot = -1;
+      prog[n - 1 - o.q0].push_back(lo);
+    }
+  }
+  P.lops.clear();
+  P.lop_begin.assign(1, 0);
+  for (int s = 0; s < n; ++s) {
+    int dl = 1, dr = 1;
+    // bond dimensions after each op, replayed in order
+    for (LocalOp &lo : prog[s]) {
+      if (lo.kind == LO_CTRL_R || lo.kind == LO_TARG_R) dr *= 2;
+      if (lo.kind == LO_CTRL_L || lo.kind == LO_TARG_L) dl *= 2;
+      lo.dl = dl;
+      lo.dr = dr;
+      P.lops.push_back(lo);
+    }
+    P.lop_begin.push_back((int)P.lops.size());
+  }
+  if (!P.grad_error.empty()) {
+    P.lops.clear();
+    P.lop_begin.assign(n + 1, 0);
+  }
+  std::vector<std::vector<int>> by_param(P.nparams);
+  for (int r = 0; r < (int)P.rot_param.size(); ++r)
+    if (P.rot_param[r] >= 0) by_param[P.rot_param[r]].push_back(r);
+  P.param_rot_begin.assign(1, 0);
+  P.param_rots.clear();
+  for (const auto &v : by_param) {
+    P.param_rots.insert(P.param_rots.end(), v.begin(), v.end());
+    P.param_rot_begin.push_back((int)P.param_rots.size());
+  }
+}
+
+// Reverse-gradient metadata of the forests: prefix leaves whose term ends use
+// a suffix-forest slot keep their environment (gleaf) so the backward pass can
+// seed that slot's adjoint with sum_k Re(c_k) conj(E_left(k)); steps are listed
+// per site for the fixed-order site-adjoint reduction.
+void build_grad_meta(Problem &P) {
+  P.ngleaf = 0;
+  std::vector<std::vector<SeedTerm>> seeds(P.nspill);
+  for (const ChainDev &ch : P.chains) {
+    if (ch.dir != 0) continue;
+    for (int st = 0; st < ch.nsteps; ++st) {
+      const StepDev &sd = P.steps[ch.step0 + st];
+      for (int c = 0; c < sd.nchild; ++c) {
+        ChildDev &cd = P.children[sd.child0 + c];
+        cd.gleaf = -1;
+        for (int t = 0; t < cd.nterm; ++t) {
+          const TermEnd &te = P.term_list[cd.term0 + t];
+          if (te.rslot < 0) continue;
+          if (cd.gleaf < 0) cd.gleaf = P.ngleaf++;
+          seeds[te.rslot].push_back(SeedTerm{te.term, cd.gleaf});
+        }
+      }
+    }
+  }
+  for (const ChainDev &ch : P.chains)
+    if (ch.dir != 0)
+      for (int st = 0; st < ch.nsteps; ++st) {
+        const StepDev &sd = P.steps[ch.step0 + st];
+        for (int c = 0; c < sd.nchild; ++c) P.children[sd.child0 + c].gleaf = -1;
+      }
+  P.seed_begin.assign(1, 0);
+  P.seed_terms.clear();
+  for (const auto &v : seeds) {
+    P.seed_terms.insert(P.seed_terms.end(), v.begin(), v.end());
+    P.seed_begin.push_back((int)P.seed_terms.size());
+  }
+  std::vector<std::vector<int>> per_site(P.n);
+  for (int s = 0; s < (int)P.steps.size(); ++s) per_site[P.steps[s].site].push_back(s);
+  P.site_steps_begin.assign(1, 0);
+  P.site_steps.clear();
+  for (const auto &v : per_site) {
+    P.site_steps.insert(P.site_steps.end(), v.begin(), v.end());
+    P.site_steps_begin.push_back((int)P.site_steps.size());
+  }
+  // forward with stored step inputs (6 products per full step) plus a
+  // backward of 8 (I/Z or X/Y only: 4 and 6)
+  P.flops_grad = 0;
+  const double prod = 8.0 * 16 * 16 * 16;
+  for (const StepDev &sd : P.steps) {
+    const bool iz = sd.need & 1, xy = sd.need & 2;
+    const int nf = 2 + (iz ? 2 : 0) + (xy ? 2 : 0);
+    const int nb = (iz && xy) ? 8 : 6;
+    P.flops_grad += (nf + nb) * prod;
+  }
+}
+
 }  // namespace
 
 std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int n_ops, int n_params,
@@ -532,6 +691,7 @@ std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int 
     throw std::runtime_error("allocated bond " + std::to_string(P->max_alloc_bond) +
                              " exceeds the 16-wide bond slots of the GPU layout");
   build_schedule(*P);
+  build_local_programs(*P, ops, n_ops);
   P->nterms = n_terms;
   P->coeffs.assign(coeffs, coeffs + 2 * (size_t)n_terms);
   P->site_words.resize(n_terms);
@@ -545,6 +705,7 @@ std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int 
     P->site_words[k] = std::move(w);
   }
   build_chains(*P);
+  build_grad_meta(*P);
   return P;
 }
 

@@ -47,6 +47,7 @@ struct Problem {
   std::vector<int> lop_begin;             // n+1
   std::vector<int> rot_param;             // per rotation id
   std::vector<double> rot_scale;
+  std::vector<int> param_rot_begin, param_rots;  // CSR: rotations of each parameter
   int ngleaf = 0;
   std::vector<int> seed_begin;            // nspill+1
   std::vector<SeedTerm> seed_terms;

@@ -20,7 +20,8 @@ class ProblemInfo(ctypes.Structure):
         'max_alloc_bond', 'n_chains', 'n_steps', 'n_spill', 'n_generations', 'n_child_nodes',
         'n_sched_steps', 'n_right_nodes')] + [
         ('flops_energy_executed', ctypes.c_double), ('flops_energy_nodes', ctypes.c_double),
-        ('flops_svd_nominal', ctypes.c_double)]
+        ('flops_svd_nominal', ctypes.c_double), ('flops_grad_executed', ctypes.c_double),
+        ('n_rotations', ctypes.c_int32), ('grad_ok', ctypes.c_int32)]
 
 
 # (name, argtypes); every function returns int status except where noted
@@ -44,6 +45,8 @@ _PROTOS = {
     'gvmps_batch_set_timing': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_get_timing': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
     'gvmps_batch_prep_stats': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
+    'gvmps_batch_value_grad': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p],
     'gvmps_batch_read_sites': [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_int)],
 }

@@ -128,6 +128,18 @@ class Batch:
             return out, t.view_as_complex(tv.view(-1, 2))
         return out
 
+    def value_grad(self, params):
+        """(energies, dE/dx) of every instance by one reverse pass over the
+        bond-doubling state (reverse_mps.py:84-99).  Gradients are returned
+        flat, n_params per instance in batch order."""
+        t = self.torch
+        x = self._params(params)
+        out = t.empty(self.size, dtype=t.float64, device=x.device)
+        g = t.empty(max(self.n_params_total, 1), dtype=t.float64, device=x.device)
+        check(lib().gvmps_batch_value_grad(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                           ctypes.c_void_p(g.data_ptr()), _stream_ptr(t)))
+        return out, g[:self.n_params_total]
+
     def _seeds(self, seeds):
         s = np.asarray(seeds, dtype=np.uint32).reshape(-1)
         if s.size == 1 and self.size > 1:

@@ -51,6 +51,9 @@ typedef struct gvmps_problem_info {
   double flops_energy_executed;    /* FP64 flops of one energy eval, as executed */
   double flops_energy_nodes;       /* prefix-node algorithmic count, 16(Dl^2Dr+DlDr^2) per node */
   double flops_svd_nominal;        /* Golub-Reinsch nominal count of all SVDs */
+  double flops_grad_executed;      /* FP64 flops of one reverse-gradient pass (forest part) */
+  int32_t n_rotations;             /* parameterised + constant rotations of the tape */
+  int32_t grad_ok;                 /* 1: the tape has a bond-doubling reverse gradient */
 } gvmps_problem_info;
 
 /* Last error message of the calling thread ("" if none). */
@@ -116,9 +119,23 @@ int gvmps_batch_sample(gvmps_batch *b, const double *params_dev, int shots, cons
 int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
                             double *energies_dev, double *terms_dev, void *stream);
 
+/* Energy and gradient dE/dx of every instance by one reverse (adjoint) pass.
+ * Replaces make_problem(kernel, H).value_gradient(x) of the reference's
+ * reverse engine (test/tools/reverse_mps.py:84-121; tape variant :141-184):
+ * the state is rebuilt with every CNOT as an exact bond doubling (no SVD,
+ * :45-50), so only neighbouring CX and single-qubit gates are accepted and
+ * the doubled bonds must fit 16; otherwise nonzero with the reason.  The
+ * loss is sum_k Re(c_k) Re<P_k> (:78-81).
+ *   energies_dev: one double per instance (optional)
+ *   grads_dev   : n_params doubles per instance, instance order (optional)
+ * Synchronises the stream. */
+int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *energies_dev,
+                           double *grads_dev, void *stream);
+
 /* Per-phase device timing with HIP events recorded on the launch stream.
- * Phases: 0 state preparation, 1 identity environments, 2 prefix chains,
- * 3 energy reduction.  enable != 0 resets the accumulators.  No reference
+ * Energy calls: 0 state preparation, 2 forest chains, 3 energy reduction.
+ * value_grad: 0 bond-doubling build, 1 forward chains, 2 energy reduction,
+ * 3 backward chains, 4 site-adjoint reduction, 5 local backward.  enable != 0 resets the accumulators.  No reference
  * counterpart (the reference times whole calls from Python,
  * test/tools/analytic_execution.py:53-62); used by the benchmark. */
 int gvmps_batch_set_timing(gvmps_batch *b, int enable);
