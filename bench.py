@@ -9,8 +9,11 @@ launch sequence.  Inputs are resident in HBM before timing starts.
 
 Printed: one JSON line (rank 0, N=1) with the batched rate as `value`, the
 plain one-circuit-per-launch rate beside it, the FP64 roofline of the
-dominant kernel (prefix chains) from HIP-event timing on the launch stream,
-and the CPU oracle timed on a bounded sample on this host.
+dominant kernel (forest chains) from HIP-event timing on the launch stream,
+and the CPU oracle timed on a bounded sample on this host.  The `adjoint`
+object reports the second half of the metric: value + dE/dx of the same 256
+vectors by one batched reverse pass per step, the plain one-vector-per-call
+rate, phase times and the torch-CPU reverse oracle on a bounded sample.
 """
 import argparse
 import json
@@ -46,6 +49,37 @@ def cpu_baseline(w, budget_s):
                        f"oracle (explicit-SVD MPS + chunk-128 prefix DAG), BLAS limited to 1 thread, {dt:.1f} s")
 
 
+def measured_traffic():
+    """HBM bytes per step of the chain kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, 'profiles', 'chains_energy_spsa40q_pmc.txt')
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('hbm_bytes_per_step:'):
+                    return float(line.split(':')[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_grad_baseline(w, budget_s):
+    """Torch-CPU restatement of the reference's reverse engine on this host."""
+    import torch
+    from oracle import reverse_ref
+    torch.set_num_threads(1)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        reverse_ref.value_gradient(w['n'], w['ops'], w['words'], w['coeffs'], w['x'][done % len(w['x'])])
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="adjoint grads/s", cores=1, kind="port",
+                sample=f"{done} value+gradient evaluation(s) (40 q, 160 params, 4000 terms) by the torch-CPU "
+                       f"restatement of the reverse engine (bond doubling, chunk 128), 1 thread, {dt:.1f} s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -54,6 +88,9 @@ def main():
     ap.add_argument('--plain-sample', type=int, default=32, help='circuits timed one per launch')
     ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU oracle work')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-grad', action='store_true')
+    ap.add_argument('--grad-steps', type=int, default=3)
+    ap.add_argument('--plain-grad-sample', type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU; --gpus must be 1",
@@ -84,27 +121,18 @@ def main():
     batched_rate = B * args.steps / dt
 
     # plain comparator: the same kernels, one circuit per launch sequence
-    single = Batch([prob], [1])
-    xs = x[: args.plain_sample]
-    single.energies(xs[0])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    plain_e = []
-    for i in range(len(xs)):
-        plain_e.append(single.energies(xs[i]))
-    torch.cuda.synchronize()
-    plain_dt = time.perf_counter() - t1
-    plain_rate = len(xs) / plain_dt
-    same = bool(torch.equal(torch.cat(plain_e), e[: len(xs)]))
+    plain_rate, same = plain_rate_of(lambda b, v: b.energies(v), Batch([prob], [1]), x[: args.plain_sample],
+                                     lambda i, r: torch.equal(r, e[i:i + 1]))
 
-    # dominant kernel: prefix chains (all generations of one step)
+    # dominant kernel: forest chains (all generations of one step)
     chains_ms = phase_ms[2] / max(calls, 1)
-    # algorithmic flops: 16 (Dl^2 Dr + Dl Dr^2) per prefix node over the term
-    # spans, plus the two identity sweeps (the planner's node count)
-    env_flops = 2.0 * w['n'] * 32.0 * 16 ** 3
+    # algorithmic flops: 16 (Dl^2 Dr + Dl Dr^2) per forest node (prefix and
+    # suffix forests, the planner's node count)
     node_flops = info['flops_energy_nodes']
     achieved = (node_flops * B) / (chains_ms * 1e-3) / 1e12
-    executed = ((info['flops_energy_executed'] - env_flops) * B) / (chains_ms * 1e-3) / 1e12
+    executed = (info['flops_energy_executed'] * B) / (chains_ms * 1e-3) / 1e12
+
+    grad = None if args.no_grad else bench_grads(args, prob, info, x, B)
     out = {
         "metric": METRIC,
         "value": round(batched_rate, 2),
@@ -121,22 +149,83 @@ def main():
         "config": {"workload": "spsa-40q: 40 qubits, 4x(RY layer + CX 39->38->...->0), chi<=16, zero cutoffs, "
                                "4000 Z-strings (weight 1-6) + constant, 256 SPSA vectors per step",
                    "batch": B, "qubits": w['n'], "terms": len(w['words']), "params": w['nparams']},
-        "plain_evals_per_s": round(plain_rate, 2),
-        "batched_vs_plain": round(batched_rate / plain_rate, 2),
+        "plain_evals_per_s": plain_rate and round(plain_rate, 2),
+        "batched_vs_plain": plain_rate and round(batched_rate / plain_rate, 2),
         "plain_matches_batched_bitwise": same,
         "phase_ms_per_step": {"state_prep": round(phase_ms[0] / max(calls, 1), 3),
-                              "identity_envs": round(phase_ms[1] / max(calls, 1), 3),
-                              "prefix_chains": round(chains_ms, 3),
+                              "forest_chains": round(chains_ms, 3),
                               "reduce": round(phase_ms[3] / max(calls, 1), 3)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_chains (all generations of one step)",
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                     "traffic": measured_traffic(),
+                     "traffic_basis": "HBM bytes per step (all generations), rocprofv3 PMC, copy-calibrated",
+                     "kernel": "k_chains_energy (all generations of one step)",
                      "algorithmic_flops_per_step": node_flops * B,
                      "executed_tflops": round(executed, 3)},
     }
+    if grad is not None:
+        out["adjoint"] = grad
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+        if grad is not None:
+            out["adjoint"]["cpu_baseline"] = cpu_grad_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def plain_rate_of(call, single, xs, matches):
+    """Rate of `call` on a one-instance batch over the vectors xs, one call
+    each, and whether every result equals the batched one; (None, None) for an
+    empty sample."""
+    import torch
+    if len(xs) == 0:
+        return None, None
+    call(single, xs[0])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    res = [call(single, xs[i]) for i in range(len(xs))]
+    torch.cuda.synchronize()
+    rate = len(xs) / (time.perf_counter() - t1)
+    return rate, all(bool(matches(i, r)) for i, r in enumerate(res))
+
+
+def bench_grads(args, prob, info, x, B):
+    """Adjoint gradients (value + dE/dx) of the same 256 vectors: one batched
+    reverse pass per step vs one vector per call."""
+    import torch
+    from gvmps import Batch
+    batch = Batch([prob], [B])
+    for _ in range(args.warmup):
+        batch.value_grad(x)
+    torch.cuda.synchronize()
+    batch.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.grad_steps):
+        e, g = batch.value_grad(x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    phase_ms, calls = batch.timing()
+    rate = B * args.grad_steps / dt
+    del batch
+    torch.cuda.empty_cache()
+    plain_rate, same = plain_rate_of(
+        lambda b, v: b.value_grad(v), Batch([prob], [1]), x[: args.plain_grad_sample],
+        lambda i, r: torch.equal(r[0], e[i:i + 1]) and torch.equal(r[1], g.view(B, -1)[i]))
+    c = max(calls, 1)
+    names = ["bond_doubling_build", "forward_chains", "energy_reduce", "backward_chains", "site_reduce",
+             "local_backward"]
+    fwd_ms, bwd_ms = phase_ms[1] / c, phase_ms[3] / c
+    flops = info['flops_grad_executed'] * B
+    achieved = flops / ((fwd_ms + bwd_ms) * 1e-3) / 1e12
+    return {
+        "value": round(rate, 2), "unit": "adjoint grads/s", "steps": args.grad_steps,
+        "ms_per_step": round(dt / args.grad_steps * 1e3, 3),
+        "plain_grads_per_s": plain_rate and round(plain_rate, 2),
+        "batched_vs_plain": plain_rate and round(rate / plain_rate, 2),
+        "plain_matches_batched_bitwise": same,
+        "phase_ms_per_step": {n: round(phase_ms[i] / c, 3) for i, n in enumerate(names)},
+        "chains_tflops": round(achieved, 3),
+        "chains_frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 4),
+    }
 
 
 if __name__ == '__main__':

@@ -1,0 +1,137 @@
+"""Host-side behaviour of the CUDA-Q-compatible surface (no GPU): kernel
+tapes, spin-operator algebra, seed mixers, bond configuration."""
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gvmps import cudaq, seeds, workloads
+from oracle import rng
+
+GOLD = json.loads((Path(__file__).parent / 'golden' / 'reference_cases.json').read_text())
+
+
+def test_kernel_tape_and_affine_parameters():
+    k, a, b = cudaq.make_kernel(float, float)
+    q = k.qalloc(3)
+    k.ry(a, q[0])
+    k.rz(2 * b - 0.5, q[1])
+    k.rx(0.25, q[2])
+    k.cx(q[0], q[1])
+    k.h(q[2])
+    assert k.width == 3 and k.argument_count == 2 and k.n_params() == 2
+    assert k.tape() == [('ry', (0,), (0, 1.0, 0.0)), ('rz', (1,), (1, 2.0, -0.5)), ('rx', (2,), 0.25),
+                        ('cx', (0, 1), None), ('h', (2,), None)]
+    with pytest.raises(TypeError):
+        k.ry(a * b, q[0])
+    with pytest.raises(TypeError):
+        k.ry(a + b, q[0])
+    with pytest.raises(ValueError):
+        k.qalloc(2)
+    with pytest.raises(RuntimeError):
+        k.cx(q[1], q[1])
+
+
+def test_list_kernel_and_argument_parsing():
+    k, thetas = cudaq.make_kernel(list)
+    q = k.qalloc(2)
+    k.ry(thetas[0], q[0])
+    k.ry(thetas[3], q[1])
+    assert k.n_params() == 4
+    assert cudaq._params(k, ([1, 2, 3, 4],)).tolist() == [1, 2, 3, 4]
+    with pytest.raises(RuntimeError):
+        cudaq._params(k, ([1, 2],))
+    f, x, y = cudaq.make_kernel(float, float)
+    assert cudaq._params(f, (0.1, 0.2)).tolist() == [0.1, 0.2]
+    assert cudaq._params(f, (np.array([0.1, 0.2]),)).tolist() == [0.1, 0.2]
+    with pytest.raises(RuntimeError):
+        cudaq._params(f, (0.1,))
+
+
+def test_spin_algebra_matches_pauli_matrices():
+    s = cudaq.spin
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0 + 0j, -1.0])
+    I2 = np.eye(2)
+    mats = {'I': I2, 'X': X, 'Y': Y, 'Z': Z}
+
+    def dense(op, n):
+        words, coeffs = op.words(n)
+        out = np.zeros((2 ** n, 2 ** n), dtype=complex)
+        for w, c in zip(words, coeffs):
+            m = np.eye(1)
+            for ch in reversed(w):  # qubit 0 is the least significant factor
+                m = np.kron(m, mats[ch])
+            out += c * m
+        return out
+
+    def kron_q(ops):  # ops[q] acting on qubit q
+        m = np.eye(1)
+        for o in reversed(ops):
+            m = np.kron(m, o)
+        return m
+
+    H = 0.5 * s.x(0) * s.y(1) + s.z(0) * s.z(0) - 2.0 * s.y(0) * s.x(0) + 3 + s.z(1) * 0.25
+    ref = 0.5 * kron_q([X, Y]) + np.eye(4) - 2.0 * kron_q([Y @ X, I2]) + 3 * np.eye(4) + 0.25 * kron_q([I2, Z])
+    np.testing.assert_allclose(dense(H, 2), ref, atol=1e-14)
+    words, coeffs = (s.x(0) * s.z(2)).words(3)
+    assert words == ['XIZ'] and coeffs.tolist() == [1.0]
+    with pytest.raises(RuntimeError):
+        s.z(3).words(3)
+    terms = []
+    H.for_each_term(lambda t: terms.append((t.to_string(False), t.get_coefficient())))
+    assert len(terms) == H.get_term_count()
+
+
+def test_seed_mixers_match_golden_and_oracle():
+    for text, h in GOLD['fnv1a32_kat']['cases']:
+        assert seeds.word_seed(0, text) == h
+    r = np.random.default_rng(3)
+    for v in r.integers(0, 2 ** 32, 100).tolist() + [seeds.INITIAL_SAMPLE_SEED]:
+        assert seeds.shot_shard_seed(v, 0) == rng.shot_shard_seed(v, 0)
+        assert seeds.word_seed(v, 'IZZI') == rng.word_seed(v, 'IZZI')
+    c = seeds.SeedCounter()
+    assert [c.next() for _ in range(3)] == [1234567, 1234568, 1234569]
+
+
+def test_configure_full_bond_cases(monkeypatch):
+    for n, pattern, reps, expected in GOLD['full_bond_caps']['cases']:
+        monkeypatch.setenv('CUDAQ_MPS_MAX_BOND', '1')
+        stages = [dict(entanglement=pattern, reps=reps)] if pattern else None
+        assert cudaq.configure_full_bond(n, stages) == expected
+        assert os.environ['CUDAQ_MPS_MAX_BOND'] == str(expected)
+        assert cudaq.engine_config() == (expected, 0.0, 0.0)
+
+
+def test_engine_config_defaults(monkeypatch):
+    for k in ('CUDAQ_MPS_MAX_BOND', 'CUDAQ_MPS_ABS_CUTOFF', 'CUDAQ_MPS_RELATIVE_CUTOFF'):
+        monkeypatch.delenv(k, raising=False)
+    assert cudaq.engine_config() == (64, 1e-5, 1e-5)
+    monkeypatch.setenv('CUDAQ_MPS_MAX_BOND', '0')
+    assert cudaq.engine_config()[0] == 1
+
+
+def test_reverse_factory_lookup():
+    k, a = cudaq.make_kernel(float)
+    q = k.qalloc(2)
+    k.ry(a, q[0])
+    assert cudaq.for_objective(lambda x: 0.0, 'off') is None
+    with pytest.raises(ValueError):
+        cudaq.for_objective(lambda x: 0.0, 'bogus')
+
+    def fun(x):
+        return 0.0
+    fun.reverse_factory = lambda: 'made'
+    assert cudaq.for_objective(fun, 'auto') == 'made'
+    assert cudaq.make_problem(object(), cudaq.spin.z(0)) is None
+
+
+def test_entangling_patterns_match_reference_definitions():
+    n = 5
+    assert workloads.entangling_pairs(n, 0, 'linear') == [(i, i + 1) for i in range(n - 1)]
+    assert workloads.entangling_pairs(n, 0, 'reverse_linear') == [(i, i + 1) for i in range(n - 2, -1, -1)]
+    assert workloads.entangling_pairs(n, 1, 'pairwise') == [(1, 2), (3, 4)]
+    assert workloads.entangling_pairs(n, 0, 'circular') == [(i, i + 1) for i in range(n - 1)] + [(n - 1, 0)]
