@@ -13,8 +13,6 @@ Modules
                 perfect sampling, shot estimator, observe/sample semantics
   dense      -- dense FP64 statevector reference (<= 24 qubits)
   reverse_ref-- bond-doubling reverse-mode MPS (torch autograd on the CPU)
-  workload   -- seeded synthetic Hamiltonians / ansatz tapes of the benchmark
-  workcount  -- algorithmic flop / byte counts of the energy path
 
 Every function cites the reference file:line it follows.  How each function
 is pinned (golden vectors, closed forms, dense statevector, brute force) is
