@@ -44,6 +44,9 @@ static T *upload(std::vector<void *> &owned, const T *host, size_t count) {
 
 Batch::~Batch() {
   if (device >= 0) hipSetDevice(device);
+  if (graph_exec) hipGraphExecDestroy(graph_exec);
+  if (graph) hipGraphDestroy(graph);
+  if (cap_stream) hipStreamDestroy(cap_stream);
   for (hipEvent_t &e : ev)
     if (e) hipEventDestroy(e);
   for (void *p : owned) hipFree(p);
@@ -204,6 +207,13 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     }
     B->items.push_back(upload(B->owned, items.data(), items.size()));
     B->n_items.push_back((int)items.size());
+    // the same work items, longest chains first across all instances
+    std::stable_sort(items.begin(), items.end(), [&](const int2 &a, const int2 &b) {
+      const int la = B->probs[B->inst[a.x].prob]->chains[a.y].nsteps;
+      const int lb = B->probs[B->inst[b.x].prob]->chains[b.y].nsteps;
+      return la > lb;
+    });
+    B->items_len.push_back(upload(B->owned, items.data(), items.size()));
   }
   return B.release();
 }
@@ -243,10 +253,10 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
     if (terms)
-      hip_check(kl_chains(ni, stream, B.items[g], B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.terms),
+      hip_check(kl_chains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.terms),
                 "k_chains launch");
     else
-      hip_check(kl_chains_energy(ni, stream, B.items[g], B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.parts),
+      hip_check(kl_chains_energy(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.parts),
                 "k_chains_energy launch");
   }
   B.mark(3, stream);
@@ -290,6 +300,51 @@ unsigned long long *launch_sample(Batch &B, const double *params, int shots, con
   return packed;
 }
 
+// Energies and, for shots > 0, samples of every instance from one state
+// preparation.  This is the sequence gvmps_batch_capture records: kernels and
+// one device copy only (no memsets, no host copies), so it replays safely.
+void launch_evaluate(Batch &B, const double *params, int shots, const unsigned *seeds_dev, double *energies_dev,
+                     unsigned char *bits, unsigned long long *packed, hipStream_t stream) {
+  const int ninst = (int)B.inst.size();
+  if (shots > 0 && (!seeds_dev || !bits || !packed))
+    throw std::runtime_error("sampling needs device seeds and both bit buffers");
+  launch_prepare(B, params, stream);
+  launch_energy_tail(B, stream, false);
+  if (energies_dev)
+    hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * ninst, hipMemcpyDeviceToDevice, stream),
+              "copy energies");
+  if (shots > 0) {
+    hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
+    hip_check(kl_sample(ninst, shots, B.max_n, stream, B.d_inst, B.arena, B.envs, seeds_dev, bits, packed),
+              "k_sample launch");
+  }
+  B.mark(5, stream);
+}
+
+void capture_evaluate(Batch &B, const double *params, int shots, const unsigned *seeds_dev, double *energies_dev,
+                      unsigned char *bits, unsigned long long *packed) {
+  if (B.graph_exec) hip_check(hipGraphExecDestroy(B.graph_exec), "hipGraphExecDestroy");
+  if (B.graph) hip_check(hipGraphDestroy(B.graph), "hipGraphDestroy");
+  B.graph_exec = nullptr;
+  B.graph = nullptr;
+  if (!B.cap_stream) hip_check(hipStreamCreateWithFlags(&B.cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+  const bool timing = B.timing;
+  B.timing = false;  // no event records inside the graph
+  hip_check(hipStreamBeginCapture(B.cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    launch_evaluate(B, params, shots, seeds_dev, energies_dev, bits, packed, B.cap_stream);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    hipStreamEndCapture(B.cap_stream, &g);
+    if (g) hipGraphDestroy(g);
+    B.timing = timing;
+    throw;
+  }
+  B.timing = timing;
+  hip_check(hipStreamEndCapture(B.cap_stream, &B.graph), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&B.graph_exec, B.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
 // Reverse gradient over the bond-doubling state.  Phases (timing slots):
 // 0 local build, 1 forward chains, 2 energy reduce, 3 backward chains,
 // 4 site-adjoint reduce, 5 local backward + parameter sums.
@@ -317,7 +372,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_gfwd(ni, stream, B.items[g], B.d_inst, B.d_probs, B.garena, B.spills, B.parts, B.estore,
+    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.spills, B.parts, B.estore,
                              B.gleafs),
               "k_chains_gfwd launch");
   }
@@ -327,7 +382,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   for (int g = B.max_gen - 1; g >= 0; --g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_bwd(ni, stream, B.items[g], B.d_inst, B.d_probs, B.garena, B.spills, B.adjspill, B.estore,
+    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.spills, B.adjspill, B.estore,
                             B.gleafs, B.aslot),
               "k_chains_bwd launch");
   }
@@ -544,6 +599,50 @@ int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *ene
                 "copy gradients");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     B.collect();
+  })
+}
+
+int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
+      throw std::runtime_error("waves per workgroup must be 1, 2 or 4");
+    if (order != 0 && order != 1) throw std::runtime_error("order must be 0 (instance-major) or 1 (longest first)");
+    B.wpb = waves_per_group;
+    B.order = order;
+  })
+}
+
+int gvmps_batch_evaluate(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds_dev,
+                         double *energies_dev, uint8_t *bits_dev, uint64_t *packed_dev, void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hipStream_t s = (hipStream_t)stream;
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    launch_evaluate(B, params_dev, shots, seeds_dev, energies_dev, bits_dev,
+                    reinterpret_cast<unsigned long long *>(packed_dev), s);
+    check_status(B, s);
+  })
+}
+
+int gvmps_batch_capture(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds_dev,
+                        double *energies_dev, uint8_t *bits_dev, uint64_t *packed_dev) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    capture_evaluate(B, params_dev, shots, seeds_dev, energies_dev, bits_dev,
+                     reinterpret_cast<unsigned long long *>(packed_dev));
+  })
+}
+
+int gvmps_batch_replay(gvmps_batch *b, void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (!B.graph_exec) throw std::runtime_error("no captured sequence (call gvmps_batch_capture first)");
+    hipStream_t s = (hipStream_t)stream;
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    hip_check(hipGraphLaunch(B.graph_exec, s), "hipGraphLaunch");
+    check_status(B, s);
   })
 }
 

@@ -32,9 +32,18 @@ struct Batch {
   unsigned long long *spacked = nullptr;
   size_t sbits_cap = 0, spacked_cap = 0;
   int max_terms = 0;
-  std::vector<int2 *> items;
+  std::vector<int2 *> items;      // per generation, instance-major
+  std::vector<int2 *> items_len;  // per generation, longest chains first across instances
   std::vector<int> n_items;
   int max_gen = 0;
+  // launch configuration of the chain kernels (set by the autotuner)
+  int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
+  int order = 0;  // 0: instance-major items, 1: longest chains first
+  const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
+  // captured evaluate + sample sequence
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   long long n_terms_total = 0, n_params_total = 0;
   int max_n = 0, max_params = 0;
   // reverse gradient (allocated on first use)
@@ -65,11 +74,11 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms);
 hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, const double *params,
                    double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
-hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                      const double2 *arena, const double2 *envs, double2 *spills, double2 *terms);
 hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
                      double *energies);
-hipError_t kl_chains_energy(int nitems, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
                             const ProbDev *probs, const double2 *arena, const double2 *envs, double2 *spills,
                             double *parts);
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
@@ -78,9 +87,9 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 // kernel launchers (kernels_grad.hip)
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const double *params, double2 *garena, double2 *rotin);
-hipError_t kl_chains_gfwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs);
-hipError_t kl_chains_bwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                          const double2 *garena, const double2 *spills, double2 *adjspill, const double2 *estore,
                          const double2 *gleafs, double2 *aslot);
 hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,

@@ -64,7 +64,7 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
-  const int w = logical * 4 + (threadIdx.x >> 6);
+  const int w = logical * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int2 it = items[w];
   const InstDev inst = insts[it.x];
@@ -251,17 +251,17 @@ hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2
   return hipGetLastError();
 }
 
-hipError_t kl_chains(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                      const double2 *arena, const double2 *envs, double2 *spills, double2 *terms) {
-  hipLaunchKernelGGL(k_chains, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, arena, envs,
+  hipLaunchKernelGGL(k_chains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena, envs,
                      spills, terms);
   return hipGetLastError();
 }
 
-hipError_t kl_chains_energy(int nitems, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
                             const ProbDev *probs, const double2 *arena, const double2 *envs, double2 *spills,
                             double *parts) {
-  hipLaunchKernelGGL(k_chains_energy, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, arena,
+  hipLaunchKernelGGL(k_chains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
                      envs, spills, parts);
   return hipGetLastError();
 }

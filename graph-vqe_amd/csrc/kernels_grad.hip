@@ -137,7 +137,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
-  const int w = logical * 4 + (threadIdx.x >> 6);
+  const int w = logical * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nitems) return;
   const int2 it = items[w];
   const InstDev inst = insts[it.x];
@@ -255,7 +255,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
-  const int w = logical * 4 + wv;
+  const int w = logical * (int)(blockDim.x >> 6) + wv;
   if (w >= nitems) return;
   const int2 it = items[w];
   const InstDev inst = insts[it.x];
@@ -500,17 +500,17 @@ hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *in
   return hipGetLastError();
 }
 
-hipError_t kl_chains_gfwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs) {
-  hipLaunchKernelGGL(k_chains_gfwd, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, garena,
+  hipLaunchKernelGGL(k_chains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, garena,
                      spills, parts, estore, gleafs);
   return hipGetLastError();
 }
 
-hipError_t kl_chains_bwd(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                          const double2 *garena, const double2 *spills, double2 *adjspill, const double2 *estore,
                          const double2 *gleafs, double2 *aslot) {
-  hipLaunchKernelGGL(k_chains_bwd, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, garena,
+  hipLaunchKernelGGL(k_chains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, garena,
                      spills, adjspill, estore, gleafs, aslot);
   return hipGetLastError();
 }

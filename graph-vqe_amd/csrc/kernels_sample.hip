@@ -110,6 +110,8 @@ k_sample(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict
   if (live && g == 0) {
     packed[((size_t)ii * shots + shot) * 2 + 0] = word0;
     packed[((size_t)ii * shots + shot) * 2 + 1] = word1;
+    // rows of smaller instances in a mixed batch: zero the padding columns
+    for (int k = n; k < stride; ++k) bits[((size_t)ii * shots + shot) * stride + k] = 0;
   }
 }
 

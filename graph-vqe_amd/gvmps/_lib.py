@@ -47,6 +47,12 @@ _PROTOS = {
     'gvmps_batch_prep_stats': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     'gvmps_batch_value_grad': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_void_p],
+    'gvmps_batch_set_launch': [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
+    'gvmps_batch_evaluate': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_batch_capture': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_batch_replay': [ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_read_sites': [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_int)],
 }

@@ -87,6 +87,18 @@ class Problem:
             self._h = None
 
 
+class CapturedEvaluation:
+    """A captured evaluate() sequence and the device buffers it is bound to."""
+
+    def __init__(self, batch, params, seeds, energies, bits, packed):
+        self.batch, self.params, self.seeds = batch, params, seeds
+        self.energies, self.bits, self.packed = energies, bits, packed
+
+    def replay(self):
+        check(lib().gvmps_batch_replay(self.batch._h, _stream_ptr(self.batch.torch)))
+        return self.energies, self.bits, self.packed
+
+
 def _stream_ptr(torch):
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -139,6 +151,52 @@ class Batch:
         check(lib().gvmps_batch_value_grad(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                            ctypes.c_void_p(g.data_ptr()), _stream_ptr(t)))
         return out, g[:self.n_params_total]
+
+    def set_launch(self, waves_per_group=4, order=0):
+        """Chain-kernel launch shape (results do not depend on it)."""
+        check(lib().gvmps_batch_set_launch(self._h, int(waves_per_group), int(order)))
+        self.launch = (int(waves_per_group), int(order))
+
+    def _eval_buffers(self, shots):
+        t = self.torch
+        dev = f'cuda:{self.device}'
+        nmax = max(p.n for p in self.problems)
+        e = t.empty(self.size, dtype=t.float64, device=dev)
+        if shots > 0:
+            bits = t.empty((self.size, int(shots), nmax), dtype=t.uint8, device=dev)
+            packed = t.empty((self.size, int(shots), 2), dtype=t.int64, device=dev)
+        else:
+            bits = packed = None
+        return e, bits, packed
+
+    def evaluate(self, params, shots=0, seeds=None):
+        """Energies and (shots > 0) samples from one state preparation:
+        (energies, bits[site order], packed)."""
+        t = self.torch
+        x = self._params(params)
+        e, bits, packed = self._eval_buffers(shots)
+        sd = None
+        if shots > 0:
+            sd = t.as_tensor(self._seeds(seeds).view(np.int32), device=x.device)  # same 32 bits
+        ptr = lambda v: ctypes.c_void_p(v.data_ptr()) if v is not None else None
+        check(lib().gvmps_batch_evaluate(self._h, ptr(x), int(shots), ptr(sd), ptr(e), ptr(bits), ptr(packed),
+                                         _stream_ptr(t)))
+        return e, bits, packed
+
+    def capture(self, params, shots=0, seeds=None):
+        """Record evaluate() into a HIP graph bound to device buffers; returns
+        a CapturedEvaluation whose replay() re-runs it (update .params /
+        .seeds in place between replays)."""
+        t = self.torch
+        x = self._params(params).clone()
+        e, bits, packed = self._eval_buffers(shots)
+        sd = None
+        if shots > 0:
+            sd = t.as_tensor(self._seeds(seeds).view(np.int32), device=x.device)  # same 32 bits
+        t.cuda.synchronize()
+        ptr = lambda v: ctypes.c_void_p(v.data_ptr()) if v is not None else None
+        check(lib().gvmps_batch_capture(self._h, ptr(x), int(shots), ptr(sd), ptr(e), ptr(bits), ptr(packed)))
+        return CapturedEvaluation(self, x, sd, e, bits, packed)
 
     def _seeds(self, seeds):
         s = np.asarray(seeds, dtype=np.uint32).reshape(-1)

@@ -132,6 +132,31 @@ int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots,
 int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *energies_dev,
                            double *grads_dev, void *stream);
 
+/* Launch configuration of the forest-chain kernels: wavefronts per workgroup
+ * (1, 2 or 4) and work-item order (0 instance-major, 1 longest chains first
+ * across instances).  Results do not depend on it.  No reference counterpart
+ * (the reference launches fixed shapes, sharded_mps.cuh:670-684); set by the
+ * autotuner (gvmps/autotune.py). */
+int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order);
+
+/* Energies plus, for shots > 0, `shots` samples per instance from ONE state
+ * preparation.  Replaces an observe() followed by a sample() on the same
+ * circuit (simulator_mps_dist.cu:1434-1485, 1587-1612), which prepares the
+ * state twice.  seeds_dev: one uint32 per instance on the device; bits_dev
+ * (instances*shots*max_n bytes, site order) and packed_dev (instances*shots*2
+ * uint64) are both required when shots > 0. */
+int gvmps_batch_evaluate(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds_dev,
+                         double *energies_dev, uint8_t *bits_dev, uint64_t *packed_dev, void *stream);
+
+/* Record gvmps_batch_evaluate's launch sequence (kernels and one device copy,
+ * no memsets) into a HIP graph bound to these device buffers; replay runs it
+ * on `stream` and checks the guard status.  Update the parameter / seed
+ * buffers in place between replays.  No reference counterpart: the
+ * reference issues every gate, SVD and chunk as separate synchronous calls. */
+int gvmps_batch_capture(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds_dev,
+                        double *energies_dev, uint8_t *bits_dev, uint64_t *packed_dev);
+int gvmps_batch_replay(gvmps_batch *b, void *stream);
+
 /* Per-phase device timing with HIP events recorded on the launch stream.
  * Energy calls: 0 state preparation, 2 forest chains, 3 energy reduction.
  * value_grad: 0 bond-doubling build, 1 forward chains, 2 energy reduction,
