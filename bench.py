@@ -89,6 +89,8 @@ def main():
     ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU oracle work')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-grad', action='store_true')
+    ap.add_argument('--no-uneven', action='store_true')
+    ap.add_argument('--uneven-reps', type=int, default=5)
     ap.add_argument('--grad-steps', type=int, default=3)
     ap.add_argument('--plain-grad-sample', type=int, default=16)
     args = ap.parse_args()
@@ -165,11 +167,58 @@ def main():
     }
     if grad is not None:
         out["adjoint"] = grad
+    if not args.no_uneven:
+        out["uneven_batch"] = bench_uneven(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
         if grad is not None:
             out["adjoint"]["cpu_baseline"] = cpu_grad_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def bench_uneven(args):
+    """64 partitions (8-32 qubits, 200-1500 terms): energies + 1000 shots each
+    from one captured HIP graph (autotuned launch shape) vs sequential
+    per-partition energy + sample calls."""
+    import numpy as np
+    import torch
+    from gvmps import Batch, Problem, autotune, seeds, workloads
+    parts = workloads.config(3)['parts']
+    probs = [Problem(q['n'], q['ops'], q['nparams'], q['words'], q['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+             for q in parts]
+    x = np.concatenate([q['x'][0] for q in parts])
+    sd = [seeds.shot_shard_seed(seeds.INITIAL_SAMPLE_SEED + j, 0) for j in range(len(parts))]
+    shots = 1000
+    batch = Batch(probs, [1] * len(probs))
+    best, table = autotune.tune(batch, x, shots, sd, reps=2, use_cache=False)
+    graph = batch.capture(x, shots, sd)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.uneven_reps):
+        graph.replay()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) / args.uneven_reps * 1e3
+    singles = [Batch([p], [1]) for p in probs]
+    for b, q, s in zip(singles, parts, sd):  # warm
+        b.energies(q['x'])
+        b.sample(q['x'], shots, [s])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    seq_e = []
+    for b, q, s in zip(singles, parts, sd):
+        seq_e.append(b.energies(q['x']))
+        b.sample(q['x'], shots, [s])
+    torch.cuda.synchronize()
+    seq_ms = (time.perf_counter() - t1) * 1e3
+    same = bool(torch.equal(torch.cat(seq_e), graph.energies))
+    ns = [int(q['n']) for q in parts]
+    return {"partitions": len(parts), "qubits": f"{min(ns)}-{max(ns)}",
+            "shots_per_partition": shots, "graph_ms": round(graph_ms, 3), "sequential_ms": round(seq_ms, 3),
+            "speedup": round(seq_ms / graph_ms, 2), "energies_match_sequential_bitwise": same,
+            "autotuned_launch": {"waves_per_group": best[0], "order": ["instance-major", "longest-first"][best[1]]},
+            "tuning_ms": {f"{k[0]}w/{'len' if k[1] else 'inst'}": round(v, 3) for k, v in table.items()},
+            "shots_per_s": round(len(parts) * shots / (graph_ms * 1e-3), 1)}
 
 
 def plain_rate_of(call, single, xs, matches):
