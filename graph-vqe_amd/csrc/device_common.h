@@ -22,6 +22,11 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // product's D registers are the next product's A operand.
 __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 
+// Chain kernels keep an environment, two site fragments and the products in
+// registers (~240 VGPRs); capping them at 256 registers gives two waves per
+// SIMD, so one wave's memory latency hides behind the other's MFMAs.
+#define GV_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }

@@ -155,14 +155,14 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
          const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, const double2 *__restrict__ envs,
          double2 *__restrict__ spills, double2 *__restrict__ terms) {
   chains_body<true>(items, nitems, insts, probs, arena, envs, spills, terms, nullptr);
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                 const ProbDev *__restrict__ probs, const double2 *__restrict__ arena,
                 const double2 *__restrict__ envs, double2 *__restrict__ spills, double *__restrict__ parts) {

@@ -129,7 +129,7 @@ k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
 // Energy chains on the bond-doubling state, keeping what the backward pass
 // reads: the input environment of every step and the prefix leaves that end
 // terms against a suffix-forest slot.
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
               const ProbDev *__restrict__ probs, const double2 *__restrict__ garena, double2 *__restrict__ spills,
               double *__restrict__ parts, double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
@@ -244,7 +244,7 @@ __device__ __forceinline__ void add_conj(Env &e, const Env &q, double w) {
 // (Hermitian pairs, Ybar10 = Ybar01^H), and with G_i = sum_j M_j Ybar_ji
 //   Ebar   = sum_i G_i M_i^H
 //   Mbar_i = 2 E G_i            (E Hermitian)
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
              const ProbDev *__restrict__ probs, const double2 *__restrict__ garena,
              const double2 *__restrict__ spills, double2 *__restrict__ adjspill,

@@ -14,6 +14,12 @@ class Op(ctypes.Structure):
                 ('param', ctypes.c_int32), ('scale', ctypes.c_double), ('offset', ctypes.c_double)]
 
 
+class Graph(ctypes.Structure):
+    _fields_ = [('n_nodes', ctypes.c_int32), ('n_edges', ctypes.c_int32), ('features', ctypes.c_void_p),
+                ('rows', ctypes.c_void_p), ('cols', ctypes.c_void_p), ('weights', ctypes.c_void_p),
+                ('rows_t', ctypes.c_void_p), ('cols_t', ctypes.c_void_p), ('weights_t', ctypes.c_void_p)]
+
+
 class ProblemInfo(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in (
         'n_qubits', 'n_params', 'n_terms', 'n_active_terms', 'n_prog_ops', 'n_two_site_gates',
@@ -53,6 +59,17 @@ _PROTOS = {
     'gvmps_batch_capture': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                             ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_replay': [ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_gnn_create': [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_uint64, ctypes.c_int],
+    'gvmps_gnn_destroy': [ctypes.c_void_p],
+    'gvmps_gnn_param_count': [ctypes.c_void_p],
+    'gvmps_gnn_params': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_gnn_forward': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_gnn_loss_grad': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_void_p],
+    'gvmps_gnn_adam': [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                       ctypes.c_void_p],
+    'gvmps_gnn_backward': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_read_sites': [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_int)],
 }

@@ -92,6 +92,91 @@ def cobyla_minimize(objective, x0, rhobeg=1.0, **options):
     return minimize(objective, x0, method='COBYLA', options=opts)
 
 
+class _RoundRobin:
+    """Gathers one value+gradient request from every running optimizer and
+    serves the round with a single batched reverse pass.  A finished
+    optimizer's instance keeps its last point (its result is discarded)."""
+
+    def __init__(self, batch, x0s):
+        import threading
+        self.batch = batch
+        self.cond = threading.Condition()
+        self.sizes = [len(x) for x in x0s]
+        self.last = [np.asarray(x, dtype=float).copy() for x in x0s]
+        self.pending, self.results = {}, {}
+        self.alive = len(x0s)
+        self.rounds = 0
+        self.error = None
+
+    def _serve(self):
+        try:
+            for i, x in self.pending.items():
+                self.last[i] = x
+            e, g = self.batch.value_grad(np.concatenate(self.last))
+            e, g = e.cpu().numpy(), g.cpu().numpy()
+            off = np.cumsum([0] + self.sizes)
+            for i in self.pending:
+                self.results[i] = (float(e[i]), g[off[i]:off[i + 1]].copy())
+        except Exception as ex:  # wake every waiter with the error
+            self.error = ex
+        self.pending = {}
+        self.rounds += 1
+        self.cond.notify_all()
+
+    def request(self, i, x):
+        with self.cond:
+            if self.error:
+                raise self.error
+            self.pending[i] = np.asarray(x, dtype=float).copy()
+            if len(self.pending) == self.alive:
+                self._serve()
+            while i not in self.results and self.error is None:
+                self.cond.wait()
+            if self.error:
+                raise self.error
+            return self.results.pop(i)
+
+    def finish(self):
+        with self.cond:
+            self.alive -= 1
+            if self.pending and len(self.pending) == self.alive:
+                self._serve()
+
+
+def lbfgs_many(problems, x0s, maxiter=200, ftol=1e-9, gtol=1e-3, maxls=40, maxcor=10):
+    """scipy L-BFGS-B on every (problem, x0) at once, with exact adjoint
+    gradients: each optimizer runs in its own thread, and all their requests
+    of one round are evaluated by one batched reverse pass.  Every iterate
+    equals that of a sequential scipy run (the batched values are bitwise the
+    single-instance values).  Returns (results, rounds)."""
+    import threading
+    from scipy.optimize import minimize
+    from .engine import Batch
+    batch = Batch(list(problems), [1] * len(problems))
+    rr = _RoundRobin(batch, x0s)
+    results = [None] * len(problems)
+    errors = []
+
+    def run(i):
+        try:
+            results[i] = minimize(lambda x: rr.request(i, x), np.asarray(x0s[i], dtype=float), jac=True,
+                                  method='L-BFGS-B',
+                                  options=dict(maxiter=maxiter, ftol=ftol, gtol=gtol, maxls=maxls, maxcor=maxcor))
+        except Exception as ex:
+            errors.append(ex)
+        finally:
+            rr.finish()
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(len(problems))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return results, rr.rounds
+
+
 def lbfgs_minimize(kernel, hamiltonian, x0, bounds=None, maxiter=100, ftol=1e-9, gtol=1e-3,
                    reverse_mode='auto', step=1e-6, callback=None):
     """L-BFGS-B with the adjoint gradient.  The first reverse energy is checked

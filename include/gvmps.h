@@ -177,6 +177,42 @@ int gvmps_batch_prep_stats(gvmps_batch *b, int *stats_host);
  * n*16*2*16 complex) and its bond dimensions (n+1 ints).  Test/debug aid. */
 int gvmps_batch_read_sites(gvmps_batch *b, int inst, double *sites_host, int *bonds_host);
 
+/* ------------------------------------------------------------------
+ * Learned warm start: a message-passing network over Hamiltonian
+ * interaction graphs, FP32, trained and run by the library's own kernels.
+ * No reference counterpart: the reference's warm start is the partition
+ * heuristic (Qiskit_VQE/independent_vqe.py:140-305), which stays the
+ * fallback.  Network: H0 = relu(X W_in^T + b), then `layers` times
+ * H_l = relu(H_{l-1} Ws^T + A_l Wn^T + b) with A_l[v] = sum_u w_uv H_{l-1}[u],
+ * then Y = H_L W_out^T + b.  Loss: mean(1 - cos(Y - target)).
+ * All graph arrays are device pointers; a batch of graphs is one graph with
+ * concatenated nodes. */
+typedef struct gvmps_gnn gvmps_gnn;
+typedef struct gvmps_graph {
+  int32_t n_nodes, n_edges;
+  const float *features;                                  /* n_nodes x in_features */
+  const int32_t *rows, *cols; const float *weights;       /* CSR by destination node */
+  const int32_t *rows_t, *cols_t; const float *weights_t; /* the same edges by source node */
+} gvmps_graph;
+
+int gvmps_gnn_create(gvmps_gnn **out, int in_features, int hidden, int layers, int outputs,
+                     uint64_t seed, int device);
+int gvmps_gnn_destroy(gvmps_gnn *m);
+int gvmps_gnn_param_count(const gvmps_gnn *m);
+/* Flat FP32 parameters: W_in, b_in, per layer (Ws, Wn, b), W_out, b_out
+ * (row-major, out x in).  Either pointer may be NULL; writing resets Adam. */
+int gvmps_gnn_params(gvmps_gnn *m, float *host_out, const float *host_in);
+int gvmps_gnn_forward(gvmps_gnn *m, const gvmps_graph *g, float *out_dev, void *stream);
+/* Forward + loss + backward into the handle's gradient; optional host copies
+ * of the loss and of the gradient (same layout as the parameters). */
+int gvmps_gnn_loss_grad(gvmps_gnn *m, const gvmps_graph *g, const float *targets_dev,
+                        float *loss_host, float *grad_host, void *stream);
+/* Forward + backward from a caller-supplied dL/dY (n_nodes x outputs, device),
+ * e.g. the MPS energy gradient with respect to the predicted angles. */
+int gvmps_gnn_backward(gvmps_gnn *m, const gvmps_graph *g, const float *dy_dev, float *grad_host,
+                       void *stream);
+int gvmps_gnn_adam(gvmps_gnn *m, float lr, float beta1, float beta2, float eps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,166 @@
+"""GPU checks of the learned warm start: the FP32 HIP network kernels against
+the float64 CPU oracle (oracle/gnn_ref.py, itself pinned against torch
+autograd), the Adam kernel against its formula, the batched L-BFGS-B driver
+against sequential scipy runs, and a small end-to-end pipeline that shows
+what the trained network gains over a random start.
+
+Tolerances: FP32 forward outputs within 2e-5 relative (inputs rounded to
+FP32 for the oracle), loss within 1e-5, gradient within 1e-4 relative norm."""
+import numpy as np
+import pytest
+
+from gvmps import gnn as G
+from gvmps import learned_warmstart as LW
+from gvmps import optimize, workloads
+from oracle import gnn_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def small_hams(count, nmin, nmax, tmin, tmax, seed):
+    return LW.hamiltonians(count, nmin, nmax, tmin, tmax, seed)
+
+
+def test_gnn_kernels_match_oracle(cuda):
+    hams = small_hams(3, 6, 10, 30, 80, 5)
+    gb = LW.graph_batch(hams)
+    net = G.WarmStartGNN(hidden=16, layers=3, outputs=4, seed=3)
+    F, H, L, O = net.shape
+    rng = np.random.default_rng(0)
+    theta = rng.normal(0, 0.4, net.P).astype(np.float32)
+    net.set_params(theta)
+    X = gb.features.cpu().numpy().astype(np.float64)
+    graph = gb.host_graph()
+    T = rng.uniform(-np.pi, np.pi, (gb.n_nodes, O)).astype(np.float32)
+    print('case: gnn forward/backward vs oracle', flush=True)
+    y = net.forward(gb).cpu().numpy()
+    yo, _ = gnn_ref.forward(theta.astype(np.float64), X, graph, F, H, L, O)
+    np.testing.assert_allclose(y, yo, rtol=2e-5, atol=2e-5 * np.abs(yo).max())
+    loss, grad = net.loss_grad(gb, T, want_grad=True)
+    lo, go = gnn_ref.loss_and_grad(theta.astype(np.float64), X, graph, T.astype(np.float64), F, H, L, O)
+    assert abs(loss - lo) < 1e-5
+    assert np.linalg.norm(grad - go) <= 1e-4 * np.linalg.norm(go)
+    dy = (np.sin(yo - T) / T.size).astype(np.float32)
+    g2 = net.backward(gb, dy, want_grad=True)
+    assert np.linalg.norm(g2 - go) <= 1e-4 * np.linalg.norm(go)
+
+
+def test_adam_step_formula(cuda):
+    hams = small_hams(2, 6, 8, 30, 60, 6)
+    gb = LW.graph_batch(hams)
+    net = G.WarmStartGNN(hidden=8, layers=2, outputs=4, seed=4)
+    T = np.zeros((gb.n_nodes, 4), dtype=np.float32)
+    th0 = net.params()
+    _, g = net.loss_grad(gb, T, want_grad=True)
+    net.adam(lr=1e-2)
+    th1 = net.params()
+    # first bias-corrected Adam step: theta - lr * g / (|g| + eps)
+    want = th0 - 1e-2 * g / (np.abs(g) + 1e-8)
+    np.testing.assert_allclose(th1, want, rtol=1e-5, atol=1e-6)
+
+
+def test_lbfgs_many_equals_sequential_scipy(cuda):
+    from scipy.optimize import minimize
+    from gvmps import Batch
+    hams = small_hams(3, 6, 9, 40, 90, 8)
+    probs = [LW.problem(*h) for h in hams]
+    x0s = [np.random.default_rng(i).uniform(-np.pi, np.pi, p.n_params) for i, p in enumerate(probs)]
+    print('case: lbfgs_many vs sequential', flush=True)
+    res, rounds = optimize.lbfgs_many(probs, x0s, maxiter=40)
+    for p, x0, r in zip(probs, x0s, res):
+        b = Batch([p], [1])
+
+        def fg(x):
+            e, g = b.value_grad(x)
+            return float(e.cpu()[0]), g.cpu().numpy()
+        s = minimize(fg, x0, jac=True, method='L-BFGS-B',
+                     options=dict(maxiter=40, ftol=1e-9, gtol=1e-3, maxls=40, maxcor=10))
+        assert s.nit == r.nit and np.array_equal(s.x, r.x) and s.fun == r.fun
+    assert rounds >= max(r.nfev for r in res)
+
+
+def test_training_and_small_pipeline_gain(cuda):
+    """Train on 48 small Hamiltonians (10-14 qubits) whose labels come from the
+    plain GPU VQE (magnetisations of the converged states), then compare
+    starts on 8 held-out 14-qubit Hamiltonians: the trained network's start
+    must have a lower energy than a random start, and the global L-BFGS-B
+    iterations from the learned and from the partition start are reported."""
+    import time
+    t0 = time.time()
+    train_h = small_hams(48, 10, 14, 80, 240, 11)
+    eval_h = small_hams(8, 14, 14, 80, 240, 12)
+    starts, _ = LW.partition_starts(train_h, 3)
+    solved = LW.solve(train_h, starts)
+    print(f'case: pipeline data ({time.time() - t0:.1f}s)', flush=True)
+    net = G.WarmStartGNN(seed=1)
+    targets = LW.magnetization_targets(train_h, [s[0] for s in solved])
+    hist = LW.train(net, train_h, targets, epochs=150, graphs_per_step=16, log_every=0)
+    assert hist[-1] < hist[0]
+    print(f'case: pipeline trained ({time.time() - t0:.1f}s), loss {hist[0]:.4f}->{hist[-1]:.4f}', flush=True)
+    learned = LW.predict_all(net, eval_h)
+    rand = [np.random.default_rng(100 + i).uniform(-np.pi, np.pi, 4 * n) for i, (n, _, _) in enumerate(eval_h)]
+    from gvmps import Batch
+    probs = [LW.problem(*h) for h in eval_h]
+    e_learned = Batch(probs, [1] * len(probs)).energies(np.concatenate(learned)).cpu().numpy()
+    e_rand = Batch(probs, [1] * len(probs)).energies(np.concatenate(rand)).cpu().numpy()
+    scale = np.array([np.abs(c).sum() for _, _, c in eval_h])
+    print(f'start energy / sum|c|: learned {np.mean(e_learned / scale):.4f}, random {np.mean(e_rand / scale):.4f}')
+    assert np.mean(e_learned / scale) < np.mean(e_rand / scale)
+    summary, _ = LW.evaluate(net, eval_h, 5)
+    print('pipeline summary', summary, flush=True)
+    assert summary['hamiltonians'] == 8 and summary['mean_nit_learned'] > 0
+
+
+def test_energy_gradient_through_network(cuda):
+    """d mean(E)/d network parameters by the adjoint kernel + network backward
+    equals a central difference of the MPS energy of the network's outputs
+    (FP32 network, so a loose bar: 2% relative on the directional derivative)."""
+    from gvmps import Batch
+    hams = small_hams(4, 6, 8, 30, 80, 21)
+    gb = LW.graph_batch(hams)
+    net = G.WarmStartGNN(hidden=16, layers=2, outputs=4, seed=9)
+    th = net.params()
+    th[-4:] = [0.3, -0.2, 0.1, 1.2]
+    net.set_params(th)
+    probs = [LW.problem(*h) for h in hams]
+    batch = Batch(probs, [1] * len(probs))
+
+    def energy_of(theta):
+        net.set_params(theta)
+        y = net.forward(gb).cpu().numpy().astype(np.float64)
+        X, off = [], 0
+        for n, _, _ in hams:
+            X.append(y[off:off + n].T.reshape(-1))
+            off += n
+        e, g = batch.value_grad(np.concatenate(X))
+        return e.cpu().numpy().mean(), g.cpu().numpy(), y
+
+    e0, g, y = energy_of(th)
+    dy, goff, noff = np.zeros_like(y), 0, 0
+    for n, _, _ in hams:
+        dy[noff:noff + n] = g[goff:goff + 4 * n].reshape(4, n).T / len(hams)
+        goff += 4 * n
+        noff += n
+    net.set_params(th)
+    grad = net.backward(gb, dy.astype(np.float32), want_grad=True).astype(np.float64)
+    d = np.random.default_rng(1).normal(size=net.P)
+    d /= np.linalg.norm(d)
+    h = 1e-2
+    fd = (energy_of((th + h * d).astype(np.float32))[0] - energy_of((th - h * d).astype(np.float32))[0]) / (2 * h)
+    assert abs(fd - grad @ d) <= 0.02 * max(abs(fd), 1e-3)
+
+
+def test_pretrained_network_beats_random_start(cuda):
+    """The shipped network (trained on 2000 Hamiltonians of 20-40 qubits)
+    gives 40-qubit starts with a lower energy than random angles."""
+    from gvmps import Batch
+    net = G.WarmStartGNN.pretrained()
+    hams = small_hams(4, 40, 40, 1000, 4000, 77)
+    probs = [LW.problem(*h) for h in hams]
+    X = LW.predict_all(net, hams)
+    R = [np.random.default_rng(i).uniform(-np.pi, np.pi, 4 * 40) for i in range(4)]
+    scale = np.array([np.abs(c).sum() for _, _, c in hams])
+    e_net = Batch(probs, [1] * 4).energies(np.concatenate(X)).cpu().numpy() / scale
+    e_rnd = Batch(probs, [1] * 4).energies(np.concatenate(R)).cpu().numpy() / scale
+    print(f'pretrained start E/sum|c| {e_net.mean():.4f} vs random {e_rnd.mean():.4f}', flush=True)
+    assert e_net.mean() < e_rnd.mean()

@@ -13,6 +13,7 @@ Each oracle function is checked against something it was not derived from:
   mps_ref sampling      dense sequential inverse-CDF with the same uniforms
   mps_ref shot energy   exact parity identity on the sampled bits
   reverse_ref           dense energies and central FD (reference test cases)
+  gnn_ref               torch autograd (float64) and central differences
   known answers         97-qubit zero-rotation energy (-0.1), cos(.4) case
 """
 import json
@@ -282,3 +283,55 @@ def test_validate_mps_six_patterns_dense():
         sim = mps_ref.RefSimulator(n, max_bond=8, abs_cut=0.0, rel_cut=0.0).run(tape)
         e, _ = sim.observe(words, coeffs)
         assert abs(e - dense.Dense(n).run(tape).energy(words, coeffs)) < 1e-12
+
+
+def _toy_graph(n, rng):
+    W = rng.uniform(0, 1, (n, n)) * (rng.uniform(0, 1, (n, n)) < 0.5)
+    np.fill_diagonal(W, 0)
+    deg = W.sum(0)
+    Wn = np.divide(W, deg[None, :], out=np.zeros_like(W), where=deg[None, :] > 0)
+    rows, cols, ws, rows_t, cols_t, ws_t = [0], [], [], [0], [], []
+    for v in range(n):
+        s = np.nonzero(Wn[:, v])[0]
+        cols += s.tolist(); ws += Wn[s, v].tolist(); rows.append(len(cols))
+    for u in range(n):
+        d = np.nonzero(Wn[u, :])[0]
+        cols_t += d.tolist(); ws_t += Wn[u, d].tolist(); rows_t.append(len(cols_t))
+    return Wn, dict(rows=rows, cols=cols, weights=ws, rows_t=rows_t, cols_t=cols_t, weights_t=ws_t)
+
+
+def test_gnn_oracle_against_torch_autograd_and_fd():
+    """oracle.gnn_ref: forward vs a torch restatement with dense adjacency,
+    hand-written backward vs torch autograd and central differences."""
+    import torch
+    from oracle import gnn_ref
+    rng = np.random.default_rng(4)
+    F, H, L, O, n = 5, 7, 3, 4, 9
+    Wn, graph = _toy_graph(n, rng)
+    _, P = gnn_ref.layout(F, H, L, O)
+    theta = rng.normal(0, 0.5, P)
+    X = rng.normal(size=(n, F))
+    T = rng.uniform(-np.pi, np.pi, (n, O))
+    loss, grad = gnn_ref.loss_and_grad(theta, X, graph, T, F, H, L, O)
+
+    th = torch.tensor(theta, requires_grad=True)
+    parts, _ = gnn_ref.layout(F, H, L, O)
+    p = {k: th[o:o + int(np.prod(s))].reshape(s) for k, (o, s) in parts.items()}
+    A = torch.tensor(Wn.T)  # A[v, u] = w_uv
+    h = torch.relu(torch.tensor(X) @ p['w_in'].T + p['b_in'])
+    for l in range(L):
+        h = torch.relu(h @ p[f'ws{l}'].T + (A @ h) @ p[f'wn{l}'].T + p[f'b{l}'])
+    y = h @ p['w_out'].T + p['b_out']
+    tl = torch.mean(1 - torch.cos(y - torch.tensor(T)))
+    tl.backward()
+    Y, _ = gnn_ref.forward(theta, X, graph, F, H, L, O)
+    np.testing.assert_allclose(Y, y.detach().numpy(), atol=1e-12)
+    assert abs(loss - float(tl)) < 1e-12
+    np.testing.assert_allclose(grad, th.grad.numpy(), atol=1e-12)
+    for k in rng.choice(P, 10, replace=False):
+        a, b = theta.copy(), theta.copy()
+        a[k] -= 1e-6
+        b[k] += 1e-6
+        fd = (gnn_ref.loss_and_grad(b, X, graph, T, F, H, L, O)[0]
+              - gnn_ref.loss_and_grad(a, X, graph, T, F, H, L, O)[0]) / 2e-6
+        assert abs(fd - grad[k]) < 1e-7
