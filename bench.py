@@ -134,6 +134,19 @@ def main():
     achieved = (node_flops * B) / (chains_ms * 1e-3) / 1e12
     executed = (info['flops_energy_executed'] * B) / (chains_ms * 1e-3) / 1e12
 
+    # the same energies with the state built by exact CNOT bond doubling (no
+    # SVD; the reverse engine's representation) instead of gates + SVDs
+    batch.set_prep('doubling')
+    e_dbl = batch.energies(x)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        e_dbl = batch.energies(x)
+    torch.cuda.synchronize()
+    dbl_rate = B * args.steps / (time.perf_counter() - t2)
+    dbl_dev = float(torch.max(torch.abs(e_dbl - e) / torch.clamp(torch.abs(e), min=1.0)))
+    batch.set_prep('svd')
+
     grad = None if args.no_grad else bench_grads(args, prob, info, x, B)
     out = {
         "metric": METRIC,
@@ -154,6 +167,8 @@ def main():
         "plain_evals_per_s": plain_rate and round(plain_rate, 2),
         "batched_vs_plain": plain_rate and round(batched_rate / plain_rate, 2),
         "plain_matches_batched_bitwise": same,
+        "doubling_prep": {"evals_per_s": round(dbl_rate, 2), "max_rel_diff_vs_svd": dbl_dev,
+                          "note": "same energies, state built by exact CNOT bond doubling instead of SVDs"},
         "phase_ms_per_step": {"state_prep": round(phase_ms[0] / max(calls, 1), 3),
                               "forest_chains": round(chains_ms, 3),
                               "reduce": round(phase_ms[3] / max(calls, 1), 3)},

@@ -239,8 +239,12 @@ void check_status(Batch &B, hipStream_t stream) {
 void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
   const int ninst = (int)B.inst.size();
   B.mark(0, stream);
-  hip_check(kl_prep(ninst, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status, B.stats),
-            "k_prep launch");
+  if (B.prep_mode == 1)
+    hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.arena, nullptr, B.status),
+              "k_build_local launch");
+  else
+    hip_check(kl_prep(ninst, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status, B.stats),
+              "k_prep launch");
   B.mark(1, stream);
 }
 
@@ -366,7 +370,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.grad_ready = true;
   }
   B.mark(0, stream);
-  hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.garena, B.rotin),
+  hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.garena, B.rotin, nullptr),
             "k_build_local launch");
   B.mark(1, stream);
   for (int g = 0; g < B.max_gen; ++g) {
@@ -610,6 +614,17 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
     if (order != 0 && order != 1) throw std::runtime_error("order must be 0 (instance-major) or 1 (longest first)");
     B.wpb = waves_per_group;
     B.order = order;
+  })
+}
+
+int gvmps_batch_set_prep(gvmps_batch *b, int mode) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (mode != 0 && mode != 1) throw std::runtime_error("preparation mode must be 0 (SVD) or 1 (bond doubling)");
+    if (mode == 1)
+      for (const Problem *P : B.probs)
+        if (!P->grad_error.empty()) throw std::runtime_error(P->grad_error);
+    B.prep_mode = mode;
   })
 }
 

@@ -39,6 +39,7 @@ struct Batch {
   // launch configuration of the chain kernels (set by the autotuner)
   int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
   int order = 0;  // 0: instance-major items, 1: longest chains first
+  int prep_mode = 0;  // 0: gates + SVDs (k_prep), 1: exact bond doubling (k_build_local)
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   // captured evaluate + sample sequence
   hipStream_t cap_stream = nullptr;
@@ -86,7 +87,7 @@ hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, cons
 void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 // kernel launchers (kernels_grad.hip)
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                          const double *params, double2 *garena, double2 *rotin);
+                          const double *params, double2 *garena, double2 *rotin, int *status);
 hipError_t kl_chains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs);
 hipError_t kl_chains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,

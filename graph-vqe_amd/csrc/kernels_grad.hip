@@ -70,11 +70,13 @@ __device__ __forceinline__ int sidx(int l, int i, int r) { return (l * 2 + i) * 
 // grid (max sites, instances), one wave per block.
 extern "C" __global__ void __launch_bounds__(64)
 k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-              const double *__restrict__ params, double2 *__restrict__ garena, double2 *__restrict__ rotin) {
+              const double *__restrict__ params, double2 *__restrict__ garena, double2 *__restrict__ rotin,
+              int *__restrict__ status) {
   __shared__ double2 buf[2][GV_SLOT];
   const int lane = threadIdx.x;
   const InstDev inst = insts[blockIdx.y];
   const int s = blockIdx.x;
+  if (status && s == 0 && lane == 0) status[blockIdx.y] = ST_OK;  // no SVD, no guard
   if (s >= inst.n) return;
   const ProbDev &P = probs[inst.prob];
   const double *x = params + inst.param_off;
@@ -90,7 +92,8 @@ k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
     const LocalOp op = P.lops[o];
     double2 U[4], dU[4];
     local_matrix(op, x, P.fixes, U, dU);
-    double2 *keep = op.kind == LO_ROT ? rotin + (inst.rot_off + op.rot) * GV_SLOT : nullptr;
+    // rotation inputs are kept only for the reverse pass (rotin == nullptr: energy-path preparation)
+    double2 *keep = (op.kind == LO_ROT && rotin != nullptr) ? rotin + (inst.rot_off + op.rot) * GV_SLOT : nullptr;
     const double2 *in = buf[cur];
     double2 *out = buf[cur ^ 1];
 #pragma unroll
@@ -495,8 +498,9 @@ k_param_grad(const InstDev *__restrict__ insts, const ProbDev *__restrict__ prob
 namespace gv {
 
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                          const double *params, double2 *garena, double2 *rotin) {
-  hipLaunchKernelGGL(k_build_local, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, garena, rotin);
+                          const double *params, double2 *garena, double2 *rotin, int *status) {
+  hipLaunchKernelGGL(k_build_local, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, garena, rotin,
+                     status);
   return hipGetLastError();
 }
 

@@ -157,6 +157,13 @@ class Batch:
         check(lib().gvmps_batch_set_launch(self._h, int(waves_per_group), int(order)))
         self.launch = (int(waves_per_group), int(order))
 
+    def set_prep(self, mode):
+        """'svd' (gates + exact-bond SVD, the plugin's method) or 'doubling'
+        (exact CNOT bond doubling, no SVD; needs a doubling-compatible tape)."""
+        m = {'svd': 0, 'doubling': 1}[mode]
+        check(lib().gvmps_batch_set_prep(self._h, m))
+        self.prep = mode
+
     def _eval_buffers(self, shots):
         t = self.torch
         dev = f'cuda:{self.device}'

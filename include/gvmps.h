@@ -139,6 +139,17 @@ int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *ene
  * autotuner (gvmps/autotune.py). */
 int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order);
 
+/* State-preparation method of the batch's energy, sample and evaluate calls:
+ * 0 (default) applies the gates with the exact-bond SVD, as the plugin does
+ * (sharded_mps.cuh:465-572, dist_svd.cuh:194-283); 1 builds the same state
+ * by exact CNOT bond doubling, the representation of the reference's reverse
+ * engine (test/tools/reverse_mps.py:45-50), with no SVD.  Mode 1 needs every
+ * problem's tape to have a doubling form within 16-wide bonds (grad_ok);
+ * energies and probabilities are those of the same state (gauge-invariant),
+ * equal to mode 0 within rounding.  read_sites then returns doubling tensors
+ * and no bond profile. */
+int gvmps_batch_set_prep(gvmps_batch *b, int mode);
+
 /* Energies plus, for shots > 0, `shots` samples per instance from ONE state
  * preparation.  Replaces an observe() followed by a sample() on the same
  * circuit (simulator_mps_dist.cu:1434-1485, 1587-1612), which prepares the

@@ -148,3 +148,24 @@ def test_spsa_40q_config_parity_and_batch_invariance(cuda):
     c = np.asarray(w['coeffs'])
     tv = tv.cpu().numpy().reshape(4, -1)
     assert np.all(np.abs((tv * c).sum(axis=1).real - e_terms) <= 1e-10 * np.abs(e_terms))
+
+
+def test_bond_doubling_preparation_matches_svd(cuda):
+    """Preparation mode 'doubling' (exact CNOT bond doubling, no SVD) gives the
+    same energies (1e-10 relative) and the same samples as the SVD path."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    X = w['x'][:4]
+    print('case: spsa-40q doubling vs svd preparation', flush=True)
+    b = Batch([p], [4])
+    e_svd = b.energies(X).cpu().numpy()
+    b.set_prep('doubling')
+    e_dbl = b.energies(X).cpu().numpy()
+    assert rel_close(e_dbl, e_svd)
+    w0 = workloads.config(0)
+    p0 = Problem(w0['n'], w0['ops'], w0['nparams'], w0['words'], w0['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    b0 = Batch([p0], [1])
+    bits_svd, _ = b0.sample(w0['x'], 1000, [12345])
+    b0.set_prep('doubling')
+    bits_dbl, _ = b0.sample(w0['x'], 1000, [12345])
+    assert np.array_equal(bits_svd.cpu().numpy(), bits_dbl.cpu().numpy())
