@@ -182,6 +182,7 @@ def main():
     }
     if grad is not None:
         out["adjoint"] = grad
+        out["adjoint_60q"] = bench_adjoint60(args)
     if not args.no_uneven:
         out["uneven_batch"] = bench_uneven(args)
     if not args.no_cpu:
@@ -189,6 +190,38 @@ def main():
         if grad is not None:
             out["adjoint"]["cpu_baseline"] = cpu_grad_baseline(w, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def bench_adjoint60(args):
+    """60 qubits, 4 layers of RY+RZ with linear CX chains (480 parameters),
+    8000 Z-strings, 32 vectors per reverse pass, step environments
+    checkpointed every other step (the rest recomputed)."""
+    import torch
+    from gvmps import Batch, Problem, workloads
+    w = workloads.config(2)
+    prob = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0.0, rel_cut=0.0)
+    info = prob.info()
+    x = torch.tensor(w['x'], dtype=torch.float64, device='cuda')
+    B = x.shape[0]
+    batch = Batch([prob], [B])
+    batch.set_checkpoint(2)
+    batch.value_grad(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.grad_steps):
+        batch.value_grad(x)
+    torch.cuda.synchronize()
+    rate = B * args.grad_steps / (time.perf_counter() - t0)
+    del batch
+    torch.cuda.empty_cache()
+    single = Batch([prob], [1])
+    single.set_checkpoint(2)
+    plain, _ = plain_rate_of(lambda b, v: b.value_grad(v), single, x[:4], lambda i, r: True)
+    env_bytes = info['n_steps'] * 4096
+    return {"workload": "adjoint-60q: 60 qubits, 4x(RY,RZ + linear CX), 480 params, 8000 Z-strings, batch 32",
+            "grads_per_s": round(rate, 2), "plain_grads_per_s": plain and round(plain, 2),
+            "batched_vs_plain": plain and round(rate / plain, 2), "checkpoint_stride": 2,
+            "step_env_bytes_per_instance_without_checkpoints": env_bytes}
 
 
 def bench_uneven(args):

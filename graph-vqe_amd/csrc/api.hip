@@ -44,6 +44,7 @@ static T *upload(std::vector<void *> &owned, const T *host, size_t count) {
 
 Batch::~Batch() {
   if (device >= 0) hipSetDevice(device);
+  if (estore) hipFree(estore);
   if (graph_exec) hipGraphExecDestroy(graph_exec);
   if (graph) hipGraphDestroy(graph);
   if (cap_stream) hipStreamDestroy(cap_stream);
@@ -126,6 +127,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.param_rots = upload(B->owned, P.param_rots.data(), P.param_rots.size());
     d.rot_scale = upload(B->owned, P.rot_scale.data(), P.rot_scale.size());
   }
+  B->h_probs = pd;
   B->d_probs = upload(B->owned, pd.data(), pd.size());
   // Z masks (site order) for the shot estimator
   std::vector<const unsigned long long *> mask_ptrs(n_problems);
@@ -215,7 +217,46 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     });
     B->items_len.push_back(upload(B->owned, items.data(), items.size()));
   }
+  B->set_checkpoint(1);
   return B.release();
+}
+
+// Checkpoint layout of the step input environments: in every chain, steps
+// 0, k, 2k, ... keep their input environment (compact slots per problem);
+// the backward pass recomputes the others from the nearest kept one.
+void Batch::set_checkpoint(int stride) {
+  if (stride < 1 || stride > 64) throw std::runtime_error("checkpoint stride must be in 1..64");
+  std::vector<long long> n_ck(probs.size(), 0);
+  ck_slots.assign(probs.size(), nullptr);
+  for (size_t j = 0; j < probs.size(); ++j) {
+    const Problem &P = *probs[j];
+    std::vector<int> cs(P.steps.size(), -1);
+    int count = 0;
+    for (const ChainDev &ch : P.chains)
+      for (int st = 0; st < ch.nsteps; ++st)
+        if (st % stride == 0) cs[ch.step0 + st] = count++;
+    n_ck[j] = count;
+    ck_slots[j] = upload(owned, cs.data(), cs.size());
+    h_probs[j].ck_slot = ck_slots[j];
+  }
+  hip_check(hipMemcpy(d_probs, h_probs.data(), sizeof(ProbDev) * h_probs.size(), hipMemcpyHostToDevice),
+            "upload problem table");
+  long long off = 0;
+  for (InstDev &in : inst) {
+    in.ck_off = off;
+    off += n_ck[in.prob];
+  }
+  hip_check(hipMemcpy(d_inst, inst.data(), sizeof(InstDev) * inst.size(), hipMemcpyHostToDevice),
+            "upload instance table");
+  n_ck_total = off;
+  ck_stride = stride;
+  if (estore) {
+    hip_check(hipFree(estore), "hipFree");
+    estore = nullptr;
+  }
+  if (grad_ready) {
+    hip_check(hipMalloc(&estore, sizeof(double2) * (size_t)std::max(1LL, n_ck_total) * GV_ENV), "hipMalloc");
+  }
 }
 
 void check_status(Batch &B, hipStream_t stream) {
@@ -362,7 +403,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.abar = upload<double2>(B.owned, nullptr, B.n_site_total);
     B.rotin = upload<double2>(B.owned, nullptr, (size_t)B.n_rot_total * GV_SLOT);
     B.rgrad = upload<double>(B.owned, nullptr, B.n_rot_total);
-    B.estore = upload<double2>(B.owned, nullptr, (size_t)B.n_steps_total * GV_ENV);
+    hip_check(hipMalloc(&B.estore, sizeof(double2) * (size_t)std::max(1LL, B.n_ck_total) * GV_ENV), "hipMalloc");
     B.aslot = upload<double2>(B.owned, nullptr, (size_t)B.n_steps_total * GV_SLOT);
     B.gleafs = upload<double2>(B.owned, nullptr, (size_t)B.n_gleaf_total * GV_ENV);
     B.adjspill = upload<double2>(B.owned, nullptr, B.n_spill_total);
@@ -614,6 +655,15 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
     if (order != 0 && order != 1) throw std::runtime_error("order must be 0 (instance-major) or 1 (longest first)");
     B.wpb = waves_per_group;
     B.order = order;
+  })
+}
+
+int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    B.set_checkpoint(stride);
   })
 }
 

@@ -50,6 +50,13 @@ struct Batch {
   // reverse gradient (allocated on first use)
   long long n_rot_total = 0, n_steps_total = 0, n_gleaf_total = 0, n_site_total = 0, n_spill_total = 0;
   bool grad_ready = false;
+  // step-environment checkpoints: every ck_stride-th step of a chain keeps
+  // its input environment, the others are recomputed in the backward pass
+  int ck_stride = 1;
+  long long n_ck_total = 0;
+  std::vector<ProbDev> h_probs;           // host copy of the problem table
+  std::vector<int *> ck_slots;            // per problem, device
+  void set_checkpoint(int stride);
   double2 *garena = nullptr, *rotin = nullptr, *estore = nullptr, *gleafs = nullptr, *adjspill = nullptr,
           *aslot = nullptr, *abar = nullptr;
   double *rgrad = nullptr, *grads = nullptr;

@@ -138,6 +138,7 @@ struct ProbDev {
   const SeedTerm* seed_terms;
   const int* site_steps_begin;  // n+1 offsets
   const int* site_steps;   // steps at each site (fixed summation order)
+  const int* ck_slot;           // per step: checkpoint slot of its input environment, -1 = recomputed
   const int* param_rot_begin;   // nparams+1 offsets: rotations driven by each parameter
   const int* param_rots;
   const double* rot_scale;      // per rotation: d angle / d parameter
@@ -159,6 +160,7 @@ struct InstDev {
   long long rot_off;   // rotation inputs (8 KiB) and rotation gradients (one double)
   long long step_off;  // step input environments (4 KiB) and step site adjoints (8 KiB)
   long long gleaf_off; // kept prefix leaf environments (4 KiB)
+  long long ck_off;    // checkpointed step input environments (4 KiB)
 };
 
 // Status codes written by kernels (per instance).

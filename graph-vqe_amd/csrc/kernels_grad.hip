@@ -148,7 +148,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   const ChainDev ch = P.chains[it.y];
   const double2 *sites = garena + inst.site_off;
   double2 *spill = spills + inst.spill_off;
-  double2 *es = estore + inst.step_off * GV_ENV;
+  double2 *es = estore + inst.ck_off * GV_ENV;
   double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
   Env e;
   if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
@@ -160,7 +160,8 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     const StepDev sd = P.steps[step];
     Frag f;
     frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
-    env_store(es + (size_t)step * GV_ENV, lane, e);
+    const int cs = P.ck_slot[step];
+    if (cs >= 0) env_store(es + (size_t)cs * GV_ENV, lane, e);
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
     prod_x(e, f, 1, x1r, x1i);
@@ -223,6 +224,46 @@ __device__ __forceinline__ void cmm(const double *ar, const double *ai, const do
   }
 }
 
+// Input environment of the step after `step` in its chain: the forward
+// transfer of `step`'s continuing child (action -2), computed with the same
+// products in the same order as k_chains_gfwd (so bit for bit the same).
+__device__ __forceinline__ void advance(const ProbDev &P, const double2 *sites, int step, bool transposed,
+                                        int lane, Env &e) {
+  const StepDev sd = P.steps[step];
+  Frag f;
+  frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+  int kind = CK_I;
+  for (int c = 0; c < sd.nchild; ++c) {
+    const ChildDev cd = P.children[sd.child0 + c];
+    if (cd.action == -2) kind = cd.kind;
+  }
+  d4 x0r, x0i, x1r, x1i, ar, ai, br, bi;
+  prod_x(e, f, 0, x0r, x0i);
+  prod_x(e, f, 1, x1r, x1i);
+  if (kind == CK_I || kind == CK_Z) {
+    prod_y(x0r, x0i, f, 0, ar, ai);  // Y00
+    prod_y(x1r, x1i, f, 1, br, bi);  // Y11
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      e.re[v] = kind == CK_I ? ar[v] + br[v] : ar[v] - br[v];
+      e.im[v] = kind == CK_I ? ai[v] + bi[v] : ai[v] - bi[v];
+    }
+  } else {
+    prod_y(x1r, x1i, f, 0, ar, ai);  // Y01 = A0^H E A1
+    prod_y(x0r, x0i, f, 1, br, bi);  // Y10 = A1^H E A0
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (kind == CK_X) {
+        e.re[v] = ar[v] + br[v];
+        e.im[v] = ai[v] + bi[v];
+      } else {
+        e.re[v] = ai[v] - bi[v];
+        e.im[v] = -ar[v] + br[v];
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void env_zero(Env &e) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) e.re[t] = e.im[t] = 0.0;
@@ -267,7 +308,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
   const double2 *sites = garena + inst.site_off;
   const double2 *spill = spills + inst.spill_off;
   double2 *aspill = adjspill + inst.spill_off;
-  const double2 *es = estore + inst.step_off * GV_ENV;
+  const double2 *es = estore + inst.ck_off * GV_ENV;
   const double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
   double2 *as = aslot + inst.step_off * GV_SLOT;
   const bool transposed = ch.dir != 0;
@@ -279,7 +320,15 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
     const int step = ch.step0 + st;
     const StepDev sd = P.steps[step];
     Env E;
-    env_load(es + (size_t)step * GV_ENV, lane, E);
+    const int cs = P.ck_slot[step];
+    if (cs >= 0) {
+      env_load(es + (size_t)cs * GV_ENV, lane, E);
+    } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
+      int s0 = st;
+      while (P.ck_slot[ch.step0 + s0] < 0) --s0;
+      env_load(es + (size_t)P.ck_slot[ch.step0 + s0] * GV_ENV, lane, E);
+      for (int q = s0; q < st; ++q) advance(P, sites, ch.step0 + q, transposed, lane, E);
+    }
     Frag fa;  // E layout of M_i (A operand form)
     frag_load(sites + (size_t)sd.site * GV_SLOT, lane, fa, !transposed);
     Env y00, y11, y01, y10;

@@ -55,6 +55,7 @@ _PROTOS = {
                                ctypes.c_void_p],
     'gvmps_batch_set_launch': [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
     'gvmps_batch_set_prep': [ctypes.c_void_p, ctypes.c_int],
+    'gvmps_batch_set_checkpoint': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_evaluate': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_capture': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

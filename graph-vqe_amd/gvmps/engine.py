@@ -157,6 +157,11 @@ class Batch:
         check(lib().gvmps_batch_set_launch(self._h, int(waves_per_group), int(order)))
         self.launch = (int(waves_per_group), int(order))
 
+    def set_checkpoint(self, stride):
+        """value_grad keeps every stride-th step environment of a chain and
+        recomputes the rest in the backward pass (same results)."""
+        check(lib().gvmps_batch_set_checkpoint(self._h, int(stride)))
+
     def set_prep(self, mode):
         """'svd' (gates + exact-bond SVD, the plugin's method) or 'doubling'
         (exact CNOT bond doubling, no SVD; needs a doubling-compatible tape)."""

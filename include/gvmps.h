@@ -132,6 +132,14 @@ int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots,
 int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *energies_dev,
                            double *grads_dev, void *stream);
 
+/* Memory/recompute trade-off of gvmps_batch_value_grad: the forward pass
+ * keeps the input environment (4 KiB) of every `stride`-th step of each chain
+ * and the backward pass recomputes the others from the nearest kept one
+ * (stride - 1 forward transfers at most).  stride 1 (default) keeps all;
+ * results are bitwise the same for every stride.  The reference's reverse
+ * engine keeps autograd's whole tape instead (test/tools/reverse_mps.py:86-99). */
+int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride);
+
 /* Launch configuration of the forest-chain kernels: wavefronts per workgroup
  * (1, 2 or 4) and work-item order (0 instance-major, 1 longest chains first
  * across instances).  Results do not depend on it.  No reference counterpart

@@ -189,3 +189,20 @@ def test_unsupported_tapes_raise(cuda):
     p2 = Problem(n, [('ry', (0,), 0), ('cx', (0, 2), None)], 1, ['ZIII'], [1.0])
     with pytest.raises(RuntimeError, match='neighbouring'):
         Batch([p2], [1]).value_grad([0.3])
+
+
+def test_checkpoint_strides_are_bitwise_identical(cuda):
+    """Keeping every 2nd / 3rd step environment and recomputing the others in
+    the backward pass gives bitwise the same energies and gradients."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], **SVD)
+    X = w['x'][:3]
+    b = Batch([p], [3])
+    print('case: spsa-40q value_grad checkpoint strides 1/2/3', flush=True)
+    e1, g1 = b.value_grad(X)
+    e1, g1 = e1.cpu().numpy(), g1.cpu().numpy()
+    for stride in (2, 3):
+        b.set_checkpoint(stride)
+        e, g = b.value_grad(X)
+        assert np.array_equal(e.cpu().numpy(), e1), stride
+        assert np.array_equal(g.cpu().numpy(), g1), stride
