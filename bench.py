@@ -176,7 +176,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                      "traffic": measured_traffic(),
                      "traffic_basis": "HBM bytes per step (all generations), rocprofv3 PMC, copy-calibrated",
-                     "kernel": "k_chains_energy (all generations of one step)",
+                     "kernel": "k_chains_energy (all generations of one step) + k_leaf_energy",
                      "algorithmic_flops_per_step": node_flops * B,
                      "executed_tflops": round(executed, 3)},
     }

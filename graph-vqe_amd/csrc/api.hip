@@ -126,6 +126,8 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.param_rot_begin = upload(B->owned, P.param_rot_begin.data(), P.param_rot_begin.size());
     d.param_rots = upload(B->owned, P.param_rots.data(), P.param_rots.size());
     d.rot_scale = upload(B->owned, P.rot_scale.data(), P.rot_scale.size());
+    d.nleaf = (int)P.leaves.size();
+    d.leaves = upload(B->owned, P.leaves.data(), P.leaves.size());
   }
   B->h_probs = pd;
   B->d_probs = upload(B->owned, pd.data(), pd.size());
@@ -147,7 +149,8 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   }
   B->d_masks = upload(B->owned, mask_ptrs.data(), mask_ptrs.size());
   // instances
-  long long site = 0, env = 0, spill = 0, term = 0, param = 0, part = 0, rot = 0, step = 0, gleaf = 0;
+  long long site = 0, env = 0, spill = 0, term = 0, param = 0, part = 0, rot = 0, step = 0, gleaf = 0, leaf = 0;
+  std::vector<int2> leaf_items;
   for (int j = 0; j < n_problems; ++j) {
     const Problem &P = *problems[j]->p;
     if (counts[j] < 0) throw std::runtime_error("negative instance count");
@@ -165,7 +168,10 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       in.rot_off = rot;
       in.step_off = step;
       in.gleaf_off = gleaf;
+      in.leaf_off = leaf;
+      for (int l = 0; l < (int)P.leaves.size(); ++l) leaf_items.push_back(make_int2((int)B->inst.size(), l));
       B->inst.push_back(in);
+      leaf += (long long)P.leaves.size();
       rot += (long long)P.rot_param.size();
       step += (long long)P.steps.size();
       gleaf += P.ngleaf;
@@ -176,7 +182,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       spill += (long long)P.nspill * GV_ENV;
       term += P.nterms;
       param += P.nparams;
-      part += (long long)P.chains.size();
+      part += (long long)P.chains.size() + (long long)P.leaves.size();
       B->max_gen = std::max(B->max_gen, P.ngen);
     }
   }
@@ -199,6 +205,9 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->energies = upload<double>(B->owned, nullptr, B->inst.size());
   B->seeds = upload<unsigned>(B->owned, nullptr, B->inst.size());
   B->parts = upload<double>(B->owned, nullptr, part);
+  B->leafs = upload<double2>(B->owned, nullptr, (size_t)leaf * GV_ENV);
+  B->leaf_items = upload(B->owned, leaf_items.data(), leaf_items.size());
+  B->n_leaf_items = (int)leaf_items.size();
   // chain work items per generation, instance-major
   for (int g = 0; g < B->max_gen; ++g) {
     std::vector<int2> items;
@@ -298,15 +307,19 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
     if (terms)
-      hip_check(kl_chains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.terms),
+      hip_check(kl_chains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.leafs, B.spills, B.terms),
                 "k_chains launch");
     else
-      hip_check(kl_chains_energy(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.envs, B.spills, B.parts),
+      hip_check(kl_chains_energy(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.leafs, B.spills,
+                                 B.parts),
                 "k_chains_energy launch");
   }
+  hip_check(kl_leaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills,
+                          terms ? B.terms : nullptr, B.parts),
+            "k_leaf_terms launch");
   B.mark(3, stream);
   if (terms) hip_check(kl_reduce(ninst, stream, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
-  else hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies), "k_reduce launch");
+  else hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 1), "k_reduce launch");
   B.mark(4, stream);
 }
 
@@ -422,7 +435,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
               "k_chains_gfwd launch");
   }
   B.mark(2, stream);
-  hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies), "k_reduce launch");
+  hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 0), "k_reduce launch");
   B.mark(3, stream);
   for (int g = B.max_gen - 1; g >= 0; --g) {
     const int ni = B.n_items[g];
@@ -444,10 +457,10 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
 }  // namespace gv
 
 // ------------------------------------------------------------------ C ABI
-#define GV_TRY(body)                        \
+#define GV_TRY(...)                         \
   try {                                     \
     g_error.clear();                        \
-    body;                                   \
+    __VA_ARGS__;                            \
     return 0;                               \
   } catch (const std::exception &ex) {      \
     g_error = ex.what();                    \
@@ -487,10 +500,23 @@ int gvmps_problem_get_generations(const gvmps_problem *p, int *out, int max_gens
     if (!p || !out) throw std::runtime_error("null pointer");
     const Problem &P = *p->p;
     for (int g = 0; g < P.ngen && g < max_gens; ++g) {
-      int steps = 0;
-      for (int c = P.gen_begin[g]; c < P.gen_begin[g + 1]; ++c) steps += P.chains[c].nsteps;
-      out[2 * g] = P.gen_begin[g + 1] - P.gen_begin[g];
-      out[2 * g + 1] = steps;
+      int steps = 0, ends = 0, spills = 0;
+      for (int c = P.gen_begin[g]; c < P.gen_begin[g + 1]; ++c) {
+        const ChainDev &ch = P.chains[c];
+        steps += ch.nsteps;
+        for (int st = 0; st < ch.nsteps; ++st) {
+          const StepDev &sd = P.steps[ch.step0 + st];
+          for (int k = 0; k < sd.nchild; ++k) {
+            const ChildDev &cd = P.children[sd.child0 + k];
+            ends += cd.nterm;
+            spills += (cd.action >= 0) + (cd.store >= 0);
+          }
+        }
+      }
+      out[4 * g] = P.gen_begin[g + 1] - P.gen_begin[g];
+      out[4 * g + 1] = steps;
+      out[4 * g + 2] = ends;
+      out[4 * g + 3] = spills;
     }
   })
 }

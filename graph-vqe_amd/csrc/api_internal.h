@@ -32,6 +32,9 @@ struct Batch {
   unsigned long long *spacked = nullptr;
   size_t sbits_cap = 0, spacked_cap = 0;
   int max_terms = 0;
+  double2 *leafs = nullptr;       // deferred term-end leaf environments
+  int2 *leaf_items = nullptr;     // (instance, leaf) work items
+  int n_leaf_items = 0;
   std::vector<int2 *> items;      // per generation, instance-major
   std::vector<int2 *> items_len;  // per generation, longest chains first across instances
   std::vector<int> n_items;
@@ -83,14 +86,16 @@ hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev
                    double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
 hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                     const double2 *arena, const double2 *envs, double2 *spills, double2 *terms);
+                     const double2 *arena, double2 *leafs, double2 *spills, double2 *terms);
+hipError_t kl_leaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                         const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
                      double *energies);
 hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                            const ProbDev *probs, const double2 *arena, const double2 *envs, double2 *spills,
+                            const ProbDev *probs, const double2 *arena, double2 *leafs, double2 *spills,
                             double *parts);
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                            const double *parts, double *energies);
+                            const double *parts, double *energies, int with_leaves);
 void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 // kernel launchers (kernels_grad.hip)
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,

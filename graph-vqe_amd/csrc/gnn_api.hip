@@ -157,10 +157,10 @@ void backward(gvmps_gnn &M, const gvmps_graph *g, const float *targets, const fl
 }
 }  // namespace
 
-#define GNN_TRY(body)                       \
+#define GNN_TRY(...)                        \
   try {                                     \
     g_gnn_error.clear();                    \
-    body;                                   \
+    __VA_ARGS__;                            \
     return 0;                               \
   } catch (const std::exception &ex) {      \
     g_gnn_error = ex.what();                \

@@ -90,6 +90,15 @@ struct ChildDev {
   int nterm;
   int store;           // suffix forest: slot receiving this node's environment, -1 none
   int gleaf;           // gradient: slot keeping this prefix leaf's environment, -1 none
+  int defer;           // energy: leaf slot whose term ends a separate pass contracts, -1 inline
+};
+
+// A prefix node with many term ends: its environment is stored in a leaf
+// slot and k_leaf_terms contracts the terms [term0, term0 + nterm) of the
+// term-end list in parallel, off the chain's critical path.
+#define GV_DEFER_TERMS 3
+struct LeafDev {
+  int term0, nterm;
 };
 
 // A term ending at a prefix-forest node: its index and the slot of its
@@ -139,6 +148,8 @@ struct ProbDev {
   const int* site_steps_begin;  // n+1 offsets
   const int* site_steps;   // steps at each site (fixed summation order)
   const int* ck_slot;           // per step: checkpoint slot of its input environment, -1 = recomputed
+  int nleaf;                    // deferred term-end leaves
+  const LeafDev* leaves;
   const int* param_rot_begin;   // nparams+1 offsets: rotations driven by each parameter
   const int* param_rots;
   const double* rot_scale;      // per rotation: d angle / d parameter
@@ -161,6 +172,7 @@ struct InstDev {
   long long step_off;  // step input environments (4 KiB) and step site adjoints (8 KiB)
   long long gleaf_off; // kept prefix leaf environments (4 KiB)
   long long ck_off;    // checkpointed step input environments (4 KiB)
+  long long leaf_off;  // deferred term-end leaf environments (4 KiB)
 };
 
 // Status codes written by kernels (per instance).

@@ -55,7 +55,7 @@ k_envs(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict__
 template <bool TERMS>
 __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int nitems,
                                             const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-                                            const double2 *__restrict__ arena, const double2 *__restrict__ envs,
+                                            const double2 *__restrict__ arena, double2 *__restrict__ leafs,
                                             double2 *__restrict__ spills, double2 *__restrict__ terms,
                                             double *__restrict__ parts) {
   const int lane = threadIdx.x & 63;
@@ -121,9 +121,12 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
       }
       if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
       // <P> = Tr(E_left R_right) = sum E[b][b'] Rt[b][b'] with the term's
-      // suffix-forest environment (or the identity right environment)
+      // suffix-forest environment (or the identity right environment);
+      // nodes with many term ends leave them to k_leaf_terms
+      if (cd.defer >= 0) env_store(leafs + (inst.leaf_off + cd.defer) * GV_ENV, lane, ce);
+      const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
 #pragma unroll 2
-      for (int t = 0; t < cd.nterm; ++t) {
+      for (int t = 0; t < nterm_inline; ++t) {
         const TermEnd te = P.term_list[cd.term0 + t];
         Env r;
         if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
@@ -157,16 +160,75 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
 
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-         const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, const double2 *__restrict__ envs,
+         const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, double2 *__restrict__ leafs,
          double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  chains_body<true>(items, nitems, insts, probs, arena, envs, spills, terms, nullptr);
+  chains_body<true>(items, nitems, insts, probs, arena, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                 const ProbDev *__restrict__ probs, const double2 *__restrict__ arena,
-                const double2 *__restrict__ envs, double2 *__restrict__ spills, double *__restrict__ parts) {
-  chains_body<false>(items, nitems, insts, probs, arena, envs, spills, nullptr, parts);
+                double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
+  chains_body<false>(items, nitems, insts, probs, arena, leafs, spills, nullptr, parts);
+}
+
+// Term ends of the deferred prefix leaves: one wave per (instance, leaf),
+// several suffix environments in flight.  TERMS: <P_k> per term; otherwise
+// one partial energy per leaf after the chain partials.
+template <bool TERMS>
+__device__ __forceinline__ void leaf_body(const int2 *__restrict__ items, int nitems,
+                                          const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                          const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
+                                          double2 *__restrict__ terms, double *__restrict__ parts) {
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nitems) return;
+  const int2 it = items[w];
+  const InstDev inst = insts[it.x];
+  const ProbDev &P = probs[inst.prob];
+  const LeafDev lf = P.leaves[it.y];
+  const double2 *spill = spills + inst.spill_off;
+  Env e;
+  env_load(leafs + (inst.leaf_off + it.y) * GV_ENV, lane, e);
+  double acc = 0.0;
+#pragma unroll 4
+  for (int t = 0; t < lf.nterm; ++t) {
+    const TermEnd te = P.term_list[lf.term0 + t];
+    Env r;
+    if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+    else env_unit(lane, r);
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sr += e.re[v] * r.re[v] - e.im[v] * r.im[v];
+      si += e.re[v] * r.im[v] + e.im[v] * r.re[v];
+    }
+    if (TERMS) {
+      sr = wave_sum(sr);
+      si = wave_sum(si);
+      if (lane == 0) terms[inst.term_off + te.term] = make_double2(sr, si);
+    } else {
+      acc += P.coeffs[2 * te.term] * sr - P.coeffs[2 * te.term + 1] * si;
+    }
+  }
+  if (!TERMS) {
+    acc = wave_sum(acc);
+    if (lane == 0) parts[inst.part_off + P.nchains + it.y] = acc;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_leaf_terms(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+             const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
+             double2 *__restrict__ terms) {
+  leaf_body<true>(items, nitems, insts, probs, leafs, spills, terms, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_leaf_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+              const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
+              double *__restrict__ parts) {
+  leaf_body<false>(items, nitems, insts, probs, leafs, spills, nullptr, parts);
 }
 
 // E = Re sum_k c_k <P_k> (+ identity words counted as <P> = 1,
@@ -213,7 +275,7 @@ k_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, d
 // of every chain, compensated and in a fixed order.
 extern "C" __global__ void __launch_bounds__(256)
 k_reduce_energy(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-                const double *__restrict__ parts, double *__restrict__ energies) {
+                const double *__restrict__ parts, double *__restrict__ energies, int with_leaves) {
   __shared__ double ssum[256], scomp[256];
   const int tid = threadIdx.x;
   const InstDev inst = insts[blockIdx.x];
@@ -226,7 +288,8 @@ k_reduce_energy(const InstDev *__restrict__ insts, const ProbDev *__restrict__ p
   };
   for (int k = tid; k < P.nterms; k += 256)
     if (!P.active[k]) add(P.coeffs[2 * k]);
-  for (int c = tid; c < P.nchains; c += 256) add(parts[inst.part_off + c]);
+  const int nparts = P.nchains + (with_leaves ? P.nleaf : 0);
+  for (int c = tid; c < nparts; c += 256) add(parts[inst.part_off + c]);
   ssum[tid] = sum;
   scomp[tid] = comp;
   __syncthreads();
@@ -252,23 +315,35 @@ hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2
 }
 
 hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                     const double2 *arena, const double2 *envs, double2 *spills, double2 *terms) {
-  hipLaunchKernelGGL(k_chains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena, envs,
-                     spills, terms);
+                     const double2 *arena, double2 *leafs, double2 *spills, double2 *terms) {
+  hipLaunchKernelGGL(k_chains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
+                     leafs, spills, terms);
+  return hipGetLastError();
+}
+
+hipError_t kl_leaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                         const double2 *leafs, const double2 *spills, double2 *terms, double *parts) {
+  if (nitems == 0) return hipSuccess;
+  if (terms)
+    hipLaunchKernelGGL(k_leaf_terms, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, leafs,
+                       spills, terms);
+  else
+    hipLaunchKernelGGL(k_leaf_energy, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, leafs,
+                       spills, parts);
   return hipGetLastError();
 }
 
 hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                            const ProbDev *probs, const double2 *arena, const double2 *envs, double2 *spills,
+                            const ProbDev *probs, const double2 *arena, double2 *leafs, double2 *spills,
                             double *parts) {
   hipLaunchKernelGGL(k_chains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
-                     envs, spills, parts);
+                     leafs, spills, parts);
   return hipGetLastError();
 }
 
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                            const double *parts, double *energies) {
-  hipLaunchKernelGGL(k_reduce_energy, dim3(ninst), dim3(256), 0, s, insts, probs, parts, energies);
+                            const double *parts, double *energies, int with_leaves) {
+  hipLaunchKernelGGL(k_reduce_energy, dim3(ninst), dim3(256), 0, s, insts, probs, parts, energies, with_leaves);
   return hipGetLastError();
 }
 

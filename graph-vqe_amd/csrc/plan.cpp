@@ -482,6 +482,7 @@ void build_chains(Problem &P) {
   P.steps.clear();
   P.children.clear();
   P.term_list.clear();
+  P.leaves.clear();
   P.gen_begin.clear();
   int last_gen = -1;
   for (int idx : order) {
@@ -500,6 +501,11 @@ void build_chains(Problem &P) {
         const int node = cdv.term0;
         cdv.term0 = (int)P.term_list.size();
         for (const TermEnd &te : F.nodes[node].ends) P.term_list.push_back(te);
+        cdv.defer = -1;
+        if (cdv.nterm >= GV_DEFER_TERMS) {
+          cdv.defer = (int)P.leaves.size();
+          P.leaves.push_back(LeafDev{cdv.term0, cdv.nterm});
+        }
         P.children.push_back(cdv);
       }
       sd.child0 = base;

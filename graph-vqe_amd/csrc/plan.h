@@ -41,6 +41,7 @@ struct Problem {
   int n_right_nodes = 0;                 // suffix-forest nodes
   std::vector<int> gen_begin;            // chains of generation g: [gen_begin[g], gen_begin[g+1])
   int nspill = 0, ngen = 0, n_nodes = 0;
+  std::vector<LeafDev> leaves;           // deferred term-end leaves (energy passes)
   // reverse gradient
   std::string grad_error;                 // non-empty: the circuit has no bond-doubling form here
   std::vector<LocalOp> lops;              // per-site local programs

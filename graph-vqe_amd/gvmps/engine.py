@@ -73,12 +73,14 @@ class Problem:
         check(lib().gvmps_problem_get_info(self._h, ctypes.byref(inf)))
         return {name: getattr(inf, name) for name, _ in inf._fields_}
 
-    def generations(self):
-        """[(chains, expansion steps)] per chain generation."""
+    def generations(self, detail=False):
+        """[(chains, expansion steps)] per chain generation; with detail,
+        (chains, steps, term ends, environment stores)."""
         g = self.info()['n_generations']
-        out = (ctypes.c_int * (2 * max(g, 1)))()
+        out = (ctypes.c_int * (4 * max(g, 1)))()
         check(lib().gvmps_problem_get_generations(self._h, out, g))
-        return [(out[2 * i], out[2 * i + 1]) for i in range(g)]
+        k = 4 if detail else 2
+        return [tuple(out[4 * i + j] for j in range(k)) for i in range(g)]
 
     def __del__(self):
         h = getattr(self, '_h', None)

@@ -71,8 +71,9 @@ int gvmps_problem_create(gvmps_problem **out, int n_qubits, const gvmps_op *ops,
                          int n_params, int max_bond, double abs_cut, double rel_cut,
                          const char *words, const double *coeffs, int n_terms);
 int gvmps_problem_destroy(gvmps_problem *p);
-/* Chain schedule per generation: out[2g] = chains, out[2g+1] = expansion
- * steps of generation g, for g < min(n_generations, max_gens).  Diagnostic. */
+/* Chain schedule per generation, 4 ints each: chains, expansion steps, term
+ * ends and environment stores of generation g, for g < min(n_generations,
+ * max_gens).  Diagnostic. */
 int gvmps_problem_get_generations(const gvmps_problem *p, int *out, int max_gens);
 int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info);
 
