@@ -33,7 +33,6 @@ struct PrepLds {
   double nrm[GV_WAVES][32];
   int perm[GV_WAVES][32];
   double2 vb[GV_WAVES][32];   // Householder vector broadcast
-  double tb[GV_WAVES];
   int bond[GV_NMAX + 1];
   int sweeps;   // Jacobi sweeps summed over the instance's gates (statistics)
   unsigned int cyc[4];  // core clocks in Theta build, QR, Jacobi, rank rule + write-back
@@ -120,73 +119,131 @@ __device__ __forceinline__ double group_sum(double v, int lpp) {
   return v;
 }
 
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_max_u64(unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(v >> 32);
+  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  return __longlong_as_double((long long)readlane_u64((unsigned long long)__double_as_longlong(v), l));
+}
+
+// Wave-uniform maximum of a 64-bit key: DPP within each row of 16 lanes
+// (quad swaps, half-row and row mirrors), then the four row maxima by readlane.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+  k = dpp_max_u64<0xB1>(k);   // quad_perm [1,0,3,2]
+  k = dpp_max_u64<0x4E>(k);   // quad_perm [2,3,0,1]
+  k = dpp_max_u64<0x141>(k);  // row_half_mirror
+  k = dpp_max_u64<0x140>(k);  // row_mirror
+  const unsigned long long a = readlane_u64(k, 0), b = readlane_u64(k, 16);
+  const unsigned long long c = readlane_u64(k, 32), d = readlane_u64(k, 48);
+  const unsigned long long ab = a > b ? a : b, cd = c > d ? c : d;
+  return ab > cd ? ab : cd;
+}
+
+// One Householder step j of the pivoted QR, j in [R0, R0 + 4).  Rows < R0 are
+// final and not touched; rows R0..j-1 of the block are masked.  Returns false
+// when the Theta block is exhausted (every remaining column norm is zero).
+template <int R0>
+__device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, int nc, int lane, double &piv0) {
+  // squared norm of rows >= j of this lane's column (four independent sums)
+  double n4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = R0; r < 32; ++r) {
+    double2 c = col[r];
+    if (r < R0 + 4 && r < j) c = make_double2(0.0, 0.0);
+    n4[r & 3] = fma(c.x, c.x, fma(c.y, c.y, n4[r & 3]));
+  }
+  const double nrm = (n4[0] + n4[1]) + (n4[2] + n4[3]);
+  // pivot: the largest norm, ties to the lowest column.  Norms are >= 0, so
+  // their bit patterns order as unsigned integers; the low 6 bits carry the
+  // column (63 - lane) instead.
+  const unsigned long long key =
+      lane < nc ? ((unsigned long long)__double_as_longlong(nrm) & ~63ULL) | (unsigned long long)(63 - lane) : 0ULL;
+  const unsigned long long best = wave_max_u64(key);
+  if ((best >> 6) == 0) return false;  // uniform
+  const int bi = 63 - (int)(best & 63);
+  const double a2 = readlane_f64(nrm, bi);  // exact squared norm of the pivot column
+  if (j == 0) piv0 = a2;
+  double2 xl = col[R0];
+#pragma unroll
+  for (int r = R0 + 1; r < R0 + 4; ++r)
+    if (r == j) xl = col[r];
+  const double2 xj = make_double2(readlane_f64(xl.x, bi), readlane_f64(xl.y, bi));
+  // v = x + e |x| e_j, e = x_j / |x_j|;  tau = 2 / (v^H v) = 1 / (|x| (|x| + |x_j|))
+  const double alpha = sqrt(a2);
+  const double axj = sqrt(xj.x * xj.x + xj.y * xj.y);
+  const double2 e = axj > 0.0 ? make_double2(xj.x / axj, xj.y / axj) : make_double2(1.0, 0.0);
+  const double tau = 1.0 / (alpha * (alpha + axj));
+  if (lane == bi) {
+#pragma unroll
+    for (int r = R0; r < 32; ++r) {
+      double2 v = col[r];
+      if (r < R0 + 4) {
+        if (r < j) v = make_double2(0.0, 0.0);
+        if (r == j) v = make_double2(xj.x + e.x * alpha, xj.y + e.y * alpha);
+      }
+      vb[r] = v;
+    }
+  }
+  wave_sync();
+  double dr[4] = {0.0, 0.0, 0.0, 0.0}, di[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = R0; r < 32; ++r) {
+    const double2 v = vb[r], c = col[r];
+    dr[r & 3] = fma(v.x, c.x, fma(v.y, c.y, dr[r & 3]));  // conj(v) c
+    di[r & 3] = fma(v.x, c.y, fma(-v.y, c.x, di[r & 3]));
+  }
+  const double dx = tau * ((dr[0] + dr[1]) + (dr[2] + dr[3]));
+  const double dy = tau * ((di[0] + di[1]) + (di[2] + di[3]));
+#pragma unroll
+  for (int r = R0; r < 32; ++r) {
+    const double2 w = vb[r];
+    col[r].x -= w.x * dx - w.y * dy;
+    col[r].y -= w.x * dy + w.y * dx;
+  }
+  wave_sync();
+  return true;
+}
+
 // Householder QR with column pivoting of the Theta block (columns < nc) of
 // the panel, applied to every panel column (so the identity block becomes
 // Q^H): panel <- Q^H panel, Q^H Theta = R P^T with |R_00| >= |R_11| >= ...
-// Lane c keeps panel column c (rows < m) in registers; the reflector of each
-// step is broadcast through `vb`.  Returns the squared Frobenius norm of the
-// rows >= `steps` of Q^H Theta (the trailing block R22).
-__device__ double wave_pivoted_qr(double2 (*W)[PW], double2 *vb, double *tb, int m, int nc, int width, int steps,
-                                  int lane, double &piv0) {
+// Lane c keeps panel column c in registers (rows >= m are zero and stay
+// zero: the reflectors vanish there); the reflector of each step is broadcast
+// through `vb`.  Returns the squared Frobenius norm of the rows >= `steps` of
+// Q^H Theta (the trailing block R22).  steps <= 16.
+__device__ double wave_pivoted_qr(double2 (*W)[PW], double2 *vb, int m, int nc, int width, int steps, int lane,
+                                  double &piv0) {
   piv0 = 0.0;
   double2 col[32];
   const bool own = lane < width;
 #pragma unroll
   for (int r = 0; r < 32; ++r) col[r] = (own && r < m) ? W[r][lane] : make_double2(0.0, 0.0);
   for (int j = 0; j < steps; ++j) {
-    double nrm = 0.0;
-#pragma unroll
-    for (int r = 0; r < 32; ++r)
-      if (r >= j && r < m) nrm += col[r].x * col[r].x + col[r].y * col[r].y;
-    double best = lane < nc ? nrm : -1.0;
-    int bi = lane;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const double ob = __shfl_xor(best, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
+    bool more;
+    switch (j >> 2) {
+      case 0: more = qr_step<0>(col, vb, j, nc, lane, piv0); break;
+      case 1: more = qr_step<4>(col, vb, j, nc, lane, piv0); break;
+      case 2: more = qr_step<8>(col, vb, j, nc, lane, piv0); break;
+      default: more = qr_step<12>(col, vb, j, nc, lane, piv0); break;
     }
-    if (!(best > 0.0)) break;  // uniform: the Theta block is exhausted
-    if (j == 0) piv0 = best;
-    if (lane == bi) {
-      // v = x + e |x| e_j, e = x_j / |x_j|;  tau = 2 / (v^H v) = 1 / (|x| (|x| + |x_j|))
-      const double alpha = sqrt(best);
-      double2 xj = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int r = 0; r < 32; ++r)
-        if (r == j) xj = col[r];
-      const double axj = sqrt(xj.x * xj.x + xj.y * xj.y);
-      const double2 e = axj > 0.0 ? make_double2(xj.x / axj, xj.y / axj) : make_double2(1.0, 0.0);
-#pragma unroll
-      for (int r = 0; r < 32; ++r)
-        if (r < m) vb[r] = r < j ? make_double2(0.0, 0.0)
-                                 : (r == j ? make_double2(xj.x + e.x * alpha, xj.y + e.y * alpha) : col[r]);
-      *tb = 1.0 / (alpha * (alpha + axj));
-    }
-    wave_sync();
-    const double tau = *tb;
-    double2 dot = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int r = 0; r < 32; ++r)
-      if (r >= j && r < m) dot = cadd(dot, cmul(cconj(vb[r]), col[r]));
-    dot.x *= tau;
-    dot.y *= tau;
-#pragma unroll
-    for (int r = 0; r < 32; ++r)
-      if (r >= j && r < m) {
-        const double2 d = cmul(vb[r], dot);
-        col[r].x -= d.x;
-        col[r].y -= d.y;
-      }
-    wave_sync();
+    if (!more) break;
   }
   double tr = 0.0;
 #pragma unroll
   for (int r = 0; r < 32; ++r)
-    if (lane < nc && r >= steps && r < m) tr += col[r].x * col[r].x + col[r].y * col[r].y;
+    if (lane < nc && r >= steps) tr += col[r].x * col[r].x + col[r].y * col[r].y;
 #pragma unroll
   for (int r = 0; r < 32; ++r)
     if (own && r < m) W[r][lane] = col[r];
@@ -329,61 +386,82 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
   const bool zero_cut = (P.abs_cut == 0.0 && P.rel_cut == 0.0);
   const int rp = min(min(count, GV_CHI), P.max_bond + (P.max_bond & 1));
   double piv0;
-  const double trail = wave_pivoted_qr(W, L.vb[w], &L.tb[w], m, nc, nc + m, rp, lane, piv0);
+  const double trail = wave_pivoted_qr(W, L.vb[w], m, nc, nc + m, rp, lane, piv0);
   const double s0_lower = sqrt(piv0);  // largest column norm <= sigma_0
   const double lim = zero_cut ? 64.0 * 2.2204460492503131e-16 * count * s0_lower
                               : fmax(P.abs_cut, P.rel_cut * s0_lower);
   const int rows = (sqrt(trail) <= lim) ? rp : m;
+  // Direct factorisation.  With zero cutoffs the rank rule keeps
+  // keep = min(max_bond, 2 Dl, 2 Dr) columns.  When the QR has revealed that
+  // rank (rows == rp == keep, ||R22||_F <= lim = 64 eps count |R_00|, and
+  // |R_00| <= sigma_0), the factorisation Q[:, :keep] (Q^H Theta)[:keep, :]
+  // and the truncated SVD U_k S_k Vh_k are both within ||R22||_F of Theta, and
+  // sigma_keep <= ||R22||_F lies below the guard scale, so the guard cannot
+  // fire.  The Jacobi sweeps would change the new bond's gauge and move the
+  // two-site block by at most 2 lim in Frobenius norm; they are skipped:
+  // left site <- Q[:, :keep] (isometry), right site <- (Q^H Theta)[:keep, :].
+  const int keep = min(min(P.max_bond, count), GV_CHI);
+  const bool direct = zero_cut && rows == rp && rp == keep;
   const unsigned long long t2 = clk();
-  const int sweeps = wave_jacobi(W, rows, nc, nc + m, lane, frob);
+  const int sweeps = direct ? 0 : wave_jacobi(W, rows, nc, nc + m, lane, frob);
   const unsigned long long t3 = clk();
   if (lane == 0) atomicAdd(&L.sweeps, sweeps);
 
-  // singular values (squared) = row norms of the Theta block
-  const int pair = lane >> 2, sub = lane & 3;
-  for (int p = pair; p < rows; p += 16) {
-    double f = 0.0;
-    for (int c = sub; c < nc; c += 4) {
-      const double2 v = W[p][c];
-      f += v.x * v.x + v.y * v.y;
-    }
-    f = quad_sum(f);
-    if (sub == 0) L.nrm[w][p] = f;
-  }
-  wave_sync();
-  if (lane < rows) {
-    const double mine = L.nrm[w][lane];
-    int rank = 0;
-    for (int q = 0; q < rows; ++q) {
-      const double o = L.nrm[w][q];
-      rank += (o > mine) || (o == mine && q < lane);
-    }
-    L.perm[w][rank] = lane;
-  }
-  wave_sync();
-  // Rank rule, evaluated identically by every lane (selectDirectSvdRank).
-  // Singular values past `rows` are bounded by sqrt(trail) <= lim: they are
-  // below the guard scale (zero cutoffs) or below the cutoff threshold.
-  const double s0 = sqrt(L.nrm[w][L.perm[w][0]]);
-  const double thr = fmax(P.abs_cut, P.rel_cut * s0);
-  const int known = min(rows, count);
-  int selected = zero_cut ? count : 0;
-  bool bad = false;
-  for (int i = 0; i < known; ++i) {
-    const double si = sqrt(L.nrm[w][L.perm[w][i]]);
-    if (!isfinite(si)) bad = true;
-    if (!zero_cut && si > thr) ++selected;
-  }
   int k;
+  bool bad = false;
+  if (direct) {
+    if (lane < rows) L.perm[w][lane] = lane;
+    wave_sync();
+    bad = !isfinite(frob);
+    k = keep;
+  } else {
+    // singular values (squared) = row norms of the Theta block
+    const int pair = lane >> 2, sub = lane & 3;
+    for (int p = pair; p < rows; p += 16) {
+      double f = 0.0;
+      for (int c = sub; c < nc; c += 4) {
+        const double2 v = W[p][c];
+        f += v.x * v.x + v.y * v.y;
+      }
+      f = quad_sum(f);
+      if (sub == 0) L.nrm[w][p] = f;
+    }
+    wave_sync();
+    if (lane < rows) {
+      const double mine = L.nrm[w][lane];
+      int rank = 0;
+      for (int q = 0; q < rows; ++q) {
+        const double o = L.nrm[w][q];
+        rank += (o > mine) || (o == mine && q < lane);
+      }
+      L.perm[w][rank] = lane;
+    }
+    wave_sync();
+    // Rank rule, evaluated identically by every lane (selectDirectSvdRank).
+    // Singular values past `rows` are bounded by sqrt(trail) <= lim: they are
+    // below the guard scale (zero cutoffs) or below the cutoff threshold.
+    const double s0 = sqrt(L.nrm[w][L.perm[w][0]]);
+    const double thr = fmax(P.abs_cut, P.rel_cut * s0);
+    const int known = min(rows, count);
+    int selected = zero_cut ? count : 0;
+    for (int i = 0; i < known; ++i) {
+      const double si = sqrt(L.nrm[w][L.perm[w][i]]);
+      if (!isfinite(si)) bad = true;
+      if (!zero_cut && si > thr) ++selected;
+    }
+    if (bad) {
+      k = 1;
+    } else if (zero_cut && P.max_bond < count && P.max_bond < rows &&
+               sqrt(L.nrm[w][L.perm[w][P.max_bond]]) > 64.0 * 2.2204460492503131e-16 * count * s0) {
+      if (lane == 0 && status) atomicMax(status, ST_GUARD);
+      k = min(P.max_bond, count);
+    } else {
+      k = max(1, min(P.max_bond, selected));
+    }
+  }
   if (bad) {
     if (lane == 0 && status) atomicMax(status, ST_NONFINITE);
     k = 1;
-  } else if (zero_cut && P.max_bond < count && P.max_bond < rows &&
-             sqrt(L.nrm[w][L.perm[w][P.max_bond]]) > 64.0 * 2.2204460492503131e-16 * count * s0) {
-    if (lane == 0 && status) atomicMax(status, ST_GUARD);
-    k = min(P.max_bond, count);
-  } else {
-    k = max(1, min(P.max_bond, selected));
   }
   k = min(min(k, rows), GV_CHI);
   double2 *wa = sites + (size_t)s * GV_SLOT;
