@@ -27,6 +27,32 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 // SIMD, so one wave's memory latency hides behind the other's MFMAs.
 #define GV_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
 
+// Index of this wavefront in its workgroup as a wave-uniform value.  Without
+// readfirstlane the compiler treats threadIdx.x >> 6 as per-lane, so every
+// work-item descriptor derived from it is fetched with vector loads into
+// VGPRs and all control flow on it runs under exec masks.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// Wave-uniform load of read-only problem data (plans, work lists, tables the
+// host uploads before the launch) through the constant address space: scalar
+// loads into SGPRs.  A plain load of the same data becomes a per-lane vector
+// load whenever the kernel also stores through other pointers, because the
+// compiler cannot prove the table is not overwritten.  The words are read as
+// ints: a struct copy would go through a generic reference and lose the
+// address space.
+template <typename T>
+__device__ __forceinline__ T uload(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "uload reads whole dwords");
+  typedef __attribute__((address_space(4))) const int cint;
+  const cint *q = (const cint *)p;
+  int w[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = q[i];
+  T v;
+  __builtin_memcpy(&v, w, sizeof(T));
+  return v;
+}
+
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -93,30 +119,36 @@ __device__ __forceinline__ void frag_load(const double2 *slot, int lane, Frag &f
     }
 }
 
+// Complex 16x16x16 products take three real MFMA products (Gauss):
+//   (a + ib)(c + id) = (ac - bd) + i((a + b)(c + d) - ac - bd),
+// 12 MFMAs instead of 16; the operand sums are four VALU adds per side.  The
+// method is normwise stable (error <= a few eps ||A|| ||B|| per entry).
+
 // X_i = E * M_i  (result in D layout: lane (j+16g), reg v = X_i[kdx(g,v)][j]).
 __device__ __forceinline__ void prod_x(const Env &e, const Frag &f, int i, d4 &xr, d4 &xi) {
-  xr = (d4){0.0, 0.0, 0.0, 0.0};
-  xi = (d4){0.0, 0.0, 0.0, 0.0};
+  d4 t1 = {0.0, 0.0, 0.0, 0.0}, t2 = {0.0, 0.0, 0.0, 0.0}, t3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    xr = mfma(e.re[t], f.re[i][t], xr);
-    xr = mfma(-e.im[t], f.im[i][t], xr);
-    xi = mfma(e.re[t], f.im[i][t], xi);
-    xi = mfma(e.im[t], f.re[i][t], xi);
+    t1 = mfma(e.re[t], f.re[i][t], t1);
+    t2 = mfma(e.im[t], f.im[i][t], t2);
+    t3 = mfma(e.re[t] + e.im[t], f.re[i][t] + f.im[i][t], t3);
   }
+  xr = t1 - t2;
+  xi = t3 - t1 - t2;
 }
 
-// Y = M_j^H X, returned in E layout (computed as Y^T = X^T conj(M_j)).
+// Y = M_j^H X, returned in E layout (computed as Y^T = X^T conj(M_j)):
+//   Re = Xr Mr + Xi Mi,  Im = Xi Mr - Xr Mi = (Xr + Xi)(Mr - Mi) - Xr Mr + Xi Mi.
 __device__ __forceinline__ void prod_y(const d4 &xr, const d4 &xi, const Frag &f, int j, d4 &yr, d4 &yi) {
-  yr = (d4){0.0, 0.0, 0.0, 0.0};
-  yi = (d4){0.0, 0.0, 0.0, 0.0};
+  d4 t1 = {0.0, 0.0, 0.0, 0.0}, t2 = {0.0, 0.0, 0.0, 0.0}, t3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    yr = mfma(xr[t], f.re[j][t], yr);
-    yr = mfma(xi[t], f.im[j][t], yr);
-    yi = mfma(xi[t], f.re[j][t], yi);
-    yi = mfma(-xr[t], f.im[j][t], yi);
+    t1 = mfma(xr[t], f.re[j][t], t1);
+    t2 = mfma(xi[t], f.im[j][t], t2);
+    t3 = mfma(xr[t] + xi[t], f.re[j][t] - f.im[j][t], t3);
   }
+  yr = t1 + t2;
+  yi = t3 - t1 + t2;
 }
 
 // Identity transfer T_I(E) = sum_i M_i^H E M_i.

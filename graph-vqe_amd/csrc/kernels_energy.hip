@@ -18,9 +18,9 @@ using namespace gv;
 extern "C" __global__ void __launch_bounds__(256)
 k_envs(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict__ arena, double2 *__restrict__ envs) {
   const int lane = threadIdx.x & 63;
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int w = blockIdx.x * 4 + wave_id();
   if (w >= 2 * ninst) return;
-  const InstDev inst = insts[w >> 1];
+  const InstDev inst = uload(insts + (w >> 1));
   const int dir = w & 1;
   const int n = inst.n;
   const double2 *sites = arena + inst.site_off;
@@ -64,12 +64,12 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
-  const int w = logical * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int w = logical * (int)(blockDim.x >> 6) + wave_id();
   if (w >= nitems) return;
-  const int2 it = items[w];
-  const InstDev inst = insts[it.x];
-  const ProbDev P = probs[inst.prob];
-  const ChainDev ch = P.chains[it.y];
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
   const int n = inst.n;
   const double2 *sites = arena + inst.site_off;
   double2 *spill = spills + inst.spill_off;
@@ -84,7 +84,7 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   const bool transposed = ch.dir != 0;
 
   double acc = 0.0;  // energy mode: this lane's share of sum_k c_k <P_k>
-  StepDev sd = P.steps[ch.step0];
+  StepDev sd = uload(P.steps + ch.step0);
   Frag f;
   frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
   for (int st = 0; st < ch.nsteps; ++st) {
@@ -92,32 +92,37 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
     StepDev nsd = sd;
     Frag nf;
     if (st + 1 < ch.nsteps) {
-      nsd = P.steps[ch.step0 + st + 1];
+      nsd = uload(P.steps + ch.step0 + st + 1);
       frag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, transposed);
     }
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
     prod_x(e, f, 1, x1r, x1i);
-    d4 y00r, y00i, y11r, y11i, y01r, y01i, y10r, y10i;
-    if (sd.need & 1) {
-      prod_y(x0r, x0i, f, 0, y00r, y00i);
-      prod_y(x1r, x1i, f, 1, y11r, y11i);
-    }
-    if (sd.need & 2) {
-      prod_y(x1r, x1i, f, 0, y01r, y01i);  // A_0^H E A_1
-      prod_y(x0r, x0i, f, 1, y10r, y10i);  // A_1^H E A_0
-    }
     Env cont;
+    // pass 0: I/Z children from Y00 = A_0^H E A_0 and Y11 = A_1^H E A_1;
+    // pass 1: X/Y children from Y01 = A_0^H E A_1 and Y10 = A_1^H E A_0
+    // (one pair of products live at a time)
+#pragma nounroll
+    for (int pass = 0; pass < 2; ++pass) {
+    if (!(sd.need & (1 << pass))) continue;
+    d4 yar, yai, ybr, ybi;
+    if (pass == 0) {
+      prod_y(x0r, x0i, f, 0, yar, yai);
+      prod_y(x1r, x1i, f, 1, ybr, ybi);
+    } else {
+      prod_y(x1r, x1i, f, 0, yar, yai);
+      prod_y(x0r, x0i, f, 1, ybr, ybi);
+    }
     for (int c = 0; c < sd.nchild; ++c) {
-      const ChildDev cd = P.children[sd.child0 + c];
+      const ChildDev cd = uload(P.children + sd.child0 + c);
+      if ((cd.kind == CK_X || cd.kind == CK_Y) != (pass == 1)) continue;
       Env ce;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         // analyticLeft phases, analytic_batch.cuh:11-14
-        if (cd.kind == CK_I) { ce.re[v] = y00r[v] + y11r[v]; ce.im[v] = y00i[v] + y11i[v]; }
-        else if (cd.kind == CK_Z) { ce.re[v] = y00r[v] - y11r[v]; ce.im[v] = y00i[v] - y11i[v]; }
-        else if (cd.kind == CK_X) { ce.re[v] = y01r[v] + y10r[v]; ce.im[v] = y01i[v] + y10i[v]; }
-        else { ce.re[v] = y01i[v] - y10i[v]; ce.im[v] = -y01r[v] + y10r[v]; }  // -i Y01 + i Y10
+        if (cd.kind == CK_I || cd.kind == CK_X) { ce.re[v] = yar[v] + ybr[v]; ce.im[v] = yai[v] + ybi[v]; }
+        else if (cd.kind == CK_Z) { ce.re[v] = yar[v] - ybr[v]; ce.im[v] = yai[v] - ybi[v]; }
+        else { ce.re[v] = yai[v] - ybi[v]; ce.im[v] = -yar[v] + ybr[v]; }  // -i Y01 + i Y10
       }
       if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
       // <P> = Tr(E_left R_right) = sum E[b][b'] Rt[b][b'] with the term's
@@ -127,7 +132,7 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
       const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
 #pragma unroll 2
       for (int t = 0; t < nterm_inline; ++t) {
-        const TermEnd te = P.term_list[cd.term0 + t];
+        const TermEnd te = uload(P.term_list + cd.term0 + t);
         Env r;
         if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
         else env_unit(lane, r);  // term ends at site n-1
@@ -142,11 +147,13 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
           si = wave_sum(si);
           if (lane == 0) tv[te.term] = make_double2(sr, si);
         } else {
-          acc += P.coeffs[2 * te.term] * sr - P.coeffs[2 * te.term + 1] * si;  // Re(c <P>)
+          const double2 cf = uload((const double2 *)P.coeffs + te.term);
+          acc += cf.x * sr - cf.y * si;  // Re(c <P>)
         }
       }
       if (cd.action == -2) cont = ce;
       else if (cd.action >= 0) env_store(spill + (size_t)cd.action * GV_ENV, lane, ce);
+    }
     }
     e = cont;
     sd = nsd;
@@ -181,19 +188,19 @@ __device__ __forceinline__ void leaf_body(const int2 *__restrict__ items, int ni
                                           const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
                                           double2 *__restrict__ terms, double *__restrict__ parts) {
   const int lane = threadIdx.x & 63;
-  const int w = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int w = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
   if (w >= nitems) return;
-  const int2 it = items[w];
-  const InstDev inst = insts[it.x];
-  const ProbDev &P = probs[inst.prob];
-  const LeafDev lf = P.leaves[it.y];
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const LeafDev lf = uload(P.leaves + it.y);
   const double2 *spill = spills + inst.spill_off;
   Env e;
   env_load(leafs + (inst.leaf_off + it.y) * GV_ENV, lane, e);
   double acc = 0.0;
 #pragma unroll 4
   for (int t = 0; t < lf.nterm; ++t) {
-    const TermEnd te = P.term_list[lf.term0 + t];
+    const TermEnd te = uload(P.term_list + lf.term0 + t);
     Env r;
     if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
     else env_unit(lane, r);
@@ -208,7 +215,8 @@ __device__ __forceinline__ void leaf_body(const int2 *__restrict__ items, int ni
       si = wave_sum(si);
       if (lane == 0) terms[inst.term_off + te.term] = make_double2(sr, si);
     } else {
-      acc += P.coeffs[2 * te.term] * sr - P.coeffs[2 * te.term + 1] * si;
+      const double2 cf = uload((const double2 *)P.coeffs + te.term);
+      acc += cf.x * sr - cf.y * si;
     }
   }
   if (!TERMS) {

@@ -140,12 +140,12 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
-  const int w = logical * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int w = logical * (int)(blockDim.x >> 6) + wave_id();
   if (w >= nitems) return;
-  const int2 it = items[w];
-  const InstDev inst = insts[it.x];
-  const ProbDev &P = probs[inst.prob];
-  const ChainDev ch = P.chains[it.y];
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
   const double2 *sites = garena + inst.site_off;
   double2 *spill = spills + inst.spill_off;
   double2 *es = estore + inst.ck_off * GV_ENV;
@@ -157,10 +157,10 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   double acc = 0.0;
   for (int st = 0; st < ch.nsteps; ++st) {
     const int step = ch.step0 + st;
-    const StepDev sd = P.steps[step];
+    const StepDev sd = uload(P.steps + step);
     Frag f;
     frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
-    const int cs = P.ck_slot[step];
+    const int cs = uload(P.ck_slot + step);
     if (cs >= 0) env_store(es + (size_t)cs * GV_ENV, lane, e);
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
@@ -176,7 +176,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     }
     Env cont;
     for (int c = 0; c < sd.nchild; ++c) {
-      const ChildDev cd = P.children[sd.child0 + c];
+      const ChildDev cd = uload(P.children + sd.child0 + c);
       Env ce;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -188,14 +188,14 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
       if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
       if (cd.gleaf >= 0) env_store(gl + (size_t)cd.gleaf * GV_ENV, lane, ce);
       for (int t = 0; t < cd.nterm; ++t) {
-        const TermEnd te = P.term_list[cd.term0 + t];
+        const TermEnd te = uload(P.term_list + cd.term0 + t);
         Env r;
         if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
         else env_unit(lane, r);
         double sr = 0.0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) sr += ce.re[v] * r.re[v] - ce.im[v] * r.im[v];
-        acc += P.coeffs[2 * te.term] * sr;  // Re(c_k) Re<P_k>
+        acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c_k) Re<P_k>
       }
       if (cd.action == -2) cont = ce;
       else if (cd.action >= 0) env_store(spill + (size_t)cd.action * GV_ENV, lane, ce);
@@ -229,12 +229,12 @@ __device__ __forceinline__ void cmm(const double *ar, const double *ai, const do
 // products in the same order as k_chains_gfwd (so bit for bit the same).
 __device__ __forceinline__ void advance(const ProbDev &P, const double2 *sites, int step, bool transposed,
                                         int lane, Env &e) {
-  const StepDev sd = P.steps[step];
+  const StepDev sd = uload(P.steps + step);
   Frag f;
   frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
   int kind = CK_I;
   for (int c = 0; c < sd.nchild; ++c) {
-    const ChildDev cd = P.children[sd.child0 + c];
+    const ChildDev cd = uload(P.children + sd.child0 + c);
     if (cd.action == -2) kind = cd.kind;
   }
   d4 x0r, x0i, x1r, x1i, ar, ai, br, bi;
@@ -295,16 +295,16 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
   __shared__ double2 tr[4][16][17];
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = wave_id();
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
   const int w = logical * (int)(blockDim.x >> 6) + wv;
   if (w >= nitems) return;
-  const int2 it = items[w];
-  const InstDev inst = insts[it.x];
-  const ProbDev &P = probs[inst.prob];
-  const ChainDev ch = P.chains[it.y];
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
   const double2 *sites = garena + inst.site_off;
   const double2 *spill = spills + inst.spill_off;
   double2 *aspill = adjspill + inst.spill_off;
@@ -318,15 +318,15 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
   env_zero(eb);
   for (int st = ch.nsteps - 1; st >= 0; --st) {
     const int step = ch.step0 + st;
-    const StepDev sd = P.steps[step];
+    const StepDev sd = uload(P.steps + step);
     Env E;
-    const int cs = P.ck_slot[step];
+    const int cs = uload(P.ck_slot + step);
     if (cs >= 0) {
       env_load(es + (size_t)cs * GV_ENV, lane, E);
     } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
       int s0 = st;
-      while (P.ck_slot[ch.step0 + s0] < 0) --s0;
-      env_load(es + (size_t)P.ck_slot[ch.step0 + s0] * GV_ENV, lane, E);
+      while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
+      env_load(es + (size_t)uload(P.ck_slot + ch.step0 + s0) * GV_ENV, lane, E);
       for (int q = s0; q < st; ++q) advance(P, sites, ch.step0 + q, transposed, lane, E);
     }
     Frag fa;  // E layout of M_i (A operand form)
@@ -337,15 +337,15 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
     env_zero(y01);
     env_zero(y10);
     for (int c = 0; c < sd.nchild; ++c) {
-      const ChildDev cd = P.children[sd.child0 + c];
+      const ChildDev cd = uload(P.children + sd.child0 + c);
       Env cb;
       if (cd.action == -2) cb = eb;
       else if (cd.action >= 0) env_load(aspill + (size_t)cd.action * GV_ENV, lane, cb);
       else env_zero(cb);
       // term ends (prefix forest): Cbar += Re(c_k) conj(Rt_k)
       for (int t = 0; t < cd.nterm; ++t) {
-        const TermEnd te = P.term_list[cd.term0 + t];
-        const double ck = P.coeffs[2 * te.term];
+        const TermEnd te = uload(P.term_list + cd.term0 + t);
+        const double ck = uload(P.coeffs + 2 * te.term);
         if (te.rslot >= 0) {
           Env r;
           env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
@@ -356,11 +356,12 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       }
       // suffix-forest slot read by prefix term ends: Cbar += Re(c_k) conj(E_left(k))
       if (cd.store >= 0) {
-        for (int q = P.seed_begin[cd.store]; q < P.seed_begin[cd.store + 1]; ++q) {
-          const SeedTerm sdt = P.seed_terms[q];
+        const int q1 = uload(P.seed_begin + cd.store + 1);
+        for (int q = uload(P.seed_begin + cd.store); q < q1; ++q) {
+          const SeedTerm sdt = uload(P.seed_terms + q);
           Env le;
           env_load(gl + (size_t)sdt.gleaf * GV_ENV, lane, le);
-          add_conj(cb, le, P.coeffs[2 * sdt.term]);
+          add_conj(cb, le, uload(P.coeffs + 2 * sdt.term));
         }
       }
 #pragma unroll
@@ -442,15 +443,16 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
 extern "C" __global__ void __launch_bounds__(256)
 k_site_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
               const double2 *__restrict__ aslot, double2 *__restrict__ abar) {
-  const InstDev inst = insts[blockIdx.y];
+  const InstDev inst = uload(insts + blockIdx.y);
   const int s = blockIdx.x;
   if (s >= inst.n) return;
-  const ProbDev &P = probs[inst.prob];
+  const ProbDev P = uload(probs + inst.prob);
   const int t = threadIdx.x;
   const double2 *as = aslot + inst.step_off * GV_SLOT;
   double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
-  for (int q = P.site_steps_begin[s]; q < P.site_steps_begin[s + 1]; ++q) {
-    const double2 *src = as + (size_t)P.site_steps[q] * GV_SLOT;
+  const int q1 = uload(P.site_steps_begin + s + 1);
+  for (int q = uload(P.site_steps_begin + s); q < q1; ++q) {
+    const double2 *src = as + (size_t)uload(P.site_steps + q) * GV_SLOT;
     const double2 a = src[2 * t], c = src[2 * t + 1];
     s0 = cadd(s0, a);
     s1 = cadd(s1, c);

@@ -77,7 +77,7 @@ __device__ __forceinline__ void rr_pair(int r, int i, int m, int &p, int &q) {
 // here once lost the imaginary part of RZ's first entry in gfx950 code.
 __device__ __forceinline__ void op_matrix(const ProgOp &o, const double *x, const double *fixes, double2 U[4]) {
   if (o.kind == OP_ROT) {
-    const double theta = (o.param >= 0 ? o.scale * x[o.param] : 0.0) + o.offset;
+    const double theta = (o.param >= 0 ? o.scale * uload(x + o.param) : 0.0) + o.offset;
     double sn, cs;
     sincos(0.5 * theta, &sn, &cs);
     const double fx = o.arg == 0 ? 1.0 : 0.0, fy = o.arg == 1 ? 1.0 : 0.0, fz = o.arg == 2 ? 1.0 : 0.0;
@@ -88,7 +88,7 @@ __device__ __forceinline__ void op_matrix(const ProgOp &o, const double *x, cons
   } else {
     const double *F = fixes + 8 * o.arg;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) U[e] = make_double2(F[2 * e], F[2 * e + 1]);
+    for (int e = 0; e < 4; ++e) U[e] = uload((const double2 *)F + e);
   }
 }
 
@@ -108,7 +108,7 @@ __device__ __forceinline__ void pending_matrix(const ProgOp *ops, int count, con
   M[3] = make_double2(1.0, 0.0);
   for (int k = 0; k < count; ++k) {
     double2 U[4];
-    op_matrix(ops[k], x, fixes, U);
+    op_matrix(uload(ops + k), x, fixes, U);
     fold(M, U);
   }
 }
@@ -346,7 +346,7 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
   double2 G[16];
   const double *gp = P.gates + 32 * gd.gate;
 #pragma unroll
-  for (int e = 0; e < 16; ++e) G[e] = make_double2(gp[2 * e], gp[2 * e + 1]);
+  for (int e = 0; e < 16; ++e) G[e] = uload((const double2 *)gp + e);
   double2(*W)[PW] = L.W[w];
   double frob = 0.0;
 #pragma unroll
@@ -488,9 +488,9 @@ extern "C" __global__ void __launch_bounds__(64 * GV_WAVES)
 k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
        double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status, int *__restrict__ stats) {
   __shared__ PrepLds L;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const InstDev inst = insts[blockIdx.x];
-  const ProbDev P = probs[inst.prob];
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const InstDev inst = uload(insts + blockIdx.x);
+  const ProbDev P = uload(probs + inst.prob);
   const int n = P.n;
   double2 *sites = arena + inst.site_off;
   const double *x = params + inst.param_off;
@@ -507,13 +507,13 @@ k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, con
   __syncthreads();
 
   for (int row = 0; row < P.nsched; ++row) {
-    const int gi = P.sched[row * GV_WAVES + w];
-    if (gi >= 0) run_gate(L, w, lane, P, P.g2[gi], x, sites, st);
+    const int gi = uload(P.sched + row * GV_WAVES + w);
+    if (gi >= 0) run_gate(L, w, lane, P, uload(P.g2 + gi), x, sites, st);
     __syncthreads();
   }
   // single-site ops left pending after the last two-site gate on each site
   for (int s = 0; s < n; ++s) {
-    const int b = P.flush[s], e = P.flush[s + 1];
+    const int b = uload(P.flush + s), e = uload(P.flush + s + 1);
     if (e == b) continue;
     double2 U[4];
     pending_matrix(P.preops + b, e - b, x, P.fixes, U);

@@ -29,14 +29,14 @@ k_sample(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict
          const double2 *__restrict__ envs, const unsigned *__restrict__ seeds, int shots, int blocks_per_inst, int stride,
          unsigned char *__restrict__ bits, unsigned long long *__restrict__ packed) {
   const int lane = threadIdx.x & 63;
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int w = blockIdx.x * 4 + wave_id();
   if (w >= ninst * blocks_per_inst) return;
   const int ii = w / blocks_per_inst, blk = w % blocks_per_inst;
-  const InstDev inst = insts[ii];
+  const InstDev inst = uload(insts + ii);
   const int n = inst.n;
   const double2 *sites = arena + inst.site_off;
   const double2 *Rt = envs + inst.env_off + (size_t)(n + 1) * GV_ENV;
-  const unsigned seed = seeds[ii];
+  const unsigned seed = uload(seeds + ii);
   const int s_loc = lane & 15, g = lane >> 4;
   const int shot = blk * 16 + s_loc;
   const bool live = shot < shots;
@@ -121,8 +121,8 @@ k_zsums(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
         const unsigned long long *__restrict__ const *masks, const unsigned long long *__restrict__ packed,
         int shots, double2 *__restrict__ terms) {
   const int ii = blockIdx.y;
-  const InstDev inst = insts[ii];
-  const ProbDev P = probs[inst.prob];
+  const InstDev inst = uload(insts + ii);
+  const ProbDev P = uload(probs + inst.prob);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.nterms) return;
   const unsigned long long m0 = masks[inst.prob][2 * k], m1 = masks[inst.prob][2 * k + 1];
