@@ -209,19 +209,36 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
 // -------------------------------------------------------------- backward
 namespace {
 
-// D += A B for complex 16x16 operands given as MFMA registers: `a` in E layout
+// Sum of complex 16x16 products in Gauss form: t1 = sum Ar Br, t2 = sum Ai Bi,
+// t3 = sum (Ar + Ai)(Br + Bi); the sum is (t1 - t2) + i (t3 - t1 - t2).
+// Three MFMA products per complex product instead of four.
+struct CAcc {
+  d4 t1, t2, t3;
+};
+
+__device__ __forceinline__ void cacc_zero(CAcc &c) {
+  c.t1 = (d4){0.0, 0.0, 0.0, 0.0};
+  c.t2 = c.t1;
+  c.t3 = c.t1;
+}
+
+// C += A B for complex 16x16 operands given as MFMA registers: `a` in E layout
 // (A operand), `b` in D layout (B operand).  ca / cb conjugate the operand.
 __device__ __forceinline__ void cmm(const double *ar, const double *ai, const double *br, const double *bi,
-                                    bool ca, bool cb, d4 &dr, d4 &di) {
+                                    bool ca, bool cb, CAcc &c) {
   const double sa = ca ? -1.0 : 1.0, sb = cb ? -1.0 : 1.0;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const double ait = sa * ai[t], bit = sb * bi[t];
-    dr = mfma(ar[t], br[t], dr);
-    dr = mfma(-ait, bit, dr);
-    di = mfma(ar[t], bit, di);
-    di = mfma(ait, br[t], di);
+    c.t1 = mfma(ar[t], br[t], c.t1);
+    c.t2 = mfma(ait, bit, c.t2);
+    c.t3 = mfma(ar[t] + ait, br[t] + bit, c.t3);
   }
+}
+
+__device__ __forceinline__ void cacc_get(const CAcc &c, d4 &dr, d4 &di) {
+  dr = c.t1 - c.t2;
+  di = c.t3 - c.t1 - c.t2;
 }
 
 // Input environment of the step after `step` in its chain: the forward
@@ -377,25 +394,35 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
     }
     // G_i in E layout, computed as G_i^T = sum_j Ybar_ji^T M_j^T with
     // Ybar00^T = conj(Ybar00), Ybar10^T = conj(Ybar01), ...
-    d4 g0r = {0, 0, 0, 0}, g0i = {0, 0, 0, 0}, g1r = {0, 0, 0, 0}, g1i = {0, 0, 0, 0};
-    if (sd.need & 1) {
-      cmm(y00.re, y00.im, fa.re[0], fa.im[0], true, false, g0r, g0i);
-      cmm(y11.re, y11.im, fa.re[1], fa.im[1], true, false, g1r, g1i);
-    }
-    if (sd.need & 2) {
-      cmm(y01.re, y01.im, fa.re[1], fa.im[1], true, false, g0r, g0i);
-      cmm(y10.re, y10.im, fa.re[0], fa.im[0], true, false, g1r, g1i);
+    d4 g0r, g0i, g1r, g1i;
+    {
+      CAcc g0, g1;
+      cacc_zero(g0);
+      cacc_zero(g1);
+      if (sd.need & 1) {
+        cmm(y00.re, y00.im, fa.re[0], fa.im[0], true, false, g0);
+        cmm(y11.re, y11.im, fa.re[1], fa.im[1], true, false, g1);
+      }
+      if (sd.need & 2) {
+        cmm(y01.re, y01.im, fa.re[1], fa.im[1], true, false, g0);
+        cmm(y10.re, y10.im, fa.re[0], fa.im[0], true, false, g1);
+      }
+      cacc_get(g0, g0r, g0i);
+      cacc_get(g1, g1r, g1i);
     }
     // Ebar^T = sum_i (G_i M_i^H)^T -> conj gives Ebar in E layout (Hermitian)
-    d4 er = {0, 0, 0, 0}, ei = {0, 0, 0, 0};
+    d4 er, ei;
     {
       double t0r[4], t0i[4], t1r[4], t1i[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         t0r[t] = g0r[t]; t0i[t] = g0i[t]; t1r[t] = g1r[t]; t1i[t] = g1i[t];
       }
-      cmm(t0r, t0i, fa.re[0], fa.im[0], false, true, er, ei);
-      cmm(t1r, t1i, fa.re[1], fa.im[1], false, true, er, ei);
+      CAcc ea;
+      cacc_zero(ea);
+      cmm(t0r, t0i, fa.re[0], fa.im[0], false, true, ea);
+      cmm(t1r, t1i, fa.re[1], fa.im[1], false, true, ea);
+      cacc_get(ea, er, ei);
     }
     // Mbar_i = 2 E G_i: G_i to D layout through LDS
     double2 *slot = as + (size_t)step * GV_SLOT;
@@ -418,8 +445,13 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      d4 mr = {0, 0, 0, 0}, mi = {0, 0, 0, 0};
-      cmm(E.re, E.im, dr, di, false, false, mr, mi);
+      d4 mr, mi;
+      {
+        CAcc ma;
+        cacc_zero(ma);
+        cmm(E.re, E.im, dr, di, false, false, ma);
+        cacc_get(ma, mr, mi);
+      }
       // D layout: lane (r + 16 g), register v = Mbar_i[kdx(g, v)][r]
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -451,7 +483,23 @@ k_site_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
   const double2 *as = aslot + inst.step_off * GV_SLOT;
   double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
   const int q1 = uload(P.site_steps_begin + s + 1);
-  for (int q = uload(P.site_steps_begin + s); q < q1; ++q) {
+  int q = uload(P.site_steps_begin + s);
+  // eight slots in flight per thread; the additions keep step order
+  for (; q + 8 <= q1; q += 8) {
+    double2 a[8], c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double2 *src = as + (size_t)uload(P.site_steps + q + u) * GV_SLOT;
+      a[u] = src[2 * t];
+      c[u] = src[2 * t + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 = cadd(s0, a[u]);
+      s1 = cadd(s1, c[u]);
+    }
+  }
+  for (; q < q1; ++q) {
     const double2 *src = as + (size_t)uload(P.site_steps + q) * GV_SLOT;
     const double2 a = src[2 * t], c = src[2 * t + 1];
     s0 = cadd(s0, a);
