@@ -3,9 +3,12 @@
 
 Workload (one step): analytic energies of 256 parameter vectors (128 SPSA
 +/- pairs) of a 40-qubit, 4-layer RY ansatz with the reversed CX chain
-(CX(39,38) ... CX(1,0)), chi <= 16, zero SVD cutoffs, complex128, on a
-4000-term Z-string Hamiltonian (weights 1-6), all 256 circuits in one batched
-launch sequence.  Inputs are resident in HBM before timing starts.
+(CX(39,38) ... CX(1,0)), chi <= 16, zero SVD cutoffs, on a 4000-term Z-string
+Hamiltonian (weights 1-6), all 256 circuits in one batched launch sequence.
+The circuit is real, so the state is exactly real and the forests run in real
+FP64 arithmetic; the complex128 kernels (the path of complex circuits) are
+timed on the same workload beside it.  Inputs are resident in HBM before
+timing starts.
 
 Printed: one JSON line (rank 0, N=1) with the batched rate as `value`, the
 plain one-circuit-per-launch rate beside it, the FP64 roofline of the
@@ -52,7 +55,7 @@ def cpu_baseline(w, budget_s):
 def measured_traffic():
     """HBM bytes per step of the chain kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
-    path = os.path.join(ROOT, 'profiles', 'chains_energy_spsa40q_pmc.txt')
+    path = os.path.join(ROOT, 'profiles', 'rchains_energy_spsa40q_pmc.txt')
     try:
         with open(path) as f:
             for line in f:
@@ -128,11 +131,38 @@ def main():
 
     # dominant kernel: forest chains (all generations of one step)
     chains_ms = phase_ms[2] / max(calls, 1)
-    # algorithmic flops: 16 (Dl^2 Dr + Dl Dr^2) per forest node (prefix and
-    # suffix forests, the planner's node count)
-    node_flops = info['flops_energy_nodes']
+    # algorithmic flops: 16 (Dl^2 Dr + Dl Dr^2) per forest node in complex
+    # arithmetic (prefix and suffix forests, the planner's node count); the
+    # circuit is real (RY + CX, Z-strings), so the forests run in real
+    # arithmetic and the count is 4 (Dl^2 Dr + Dl Dr^2): a quarter
+    real = bool(info['real_arith'])
+    arith_div = 4.0 if real else 1.0
+    node_flops = info['flops_energy_nodes'] / arith_div
     achieved = (node_flops * B) / (chains_ms * 1e-3) / 1e12
-    executed = (info['flops_energy_executed'] * B) / (chains_ms * 1e-3) / 1e12
+    executed = (info['flops_energy_executed'] / arith_div * B) / (chains_ms * 1e-3) / 1e12
+
+    # the same workload on the complex kernels (the path of complex circuits)
+    complex_leg = None
+    if real:
+        batch.set_arith('complex')
+        e_cx = batch.energies(x)
+        torch.cuda.synchronize()
+        batch.set_timing(False)
+        batch.set_timing(True)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            e_cx = batch.energies(x)
+        torch.cuda.synchronize()
+        cx_rate = B * args.steps / (time.perf_counter() - t1)
+        cx_ms, cx_calls = batch.timing()
+        cx_chains = cx_ms[2] / max(cx_calls, 1)
+        cx_flops = info['flops_energy_nodes'] * B / (cx_chains * 1e-3) / 1e12
+        complex_leg = {"evals_per_s": round(cx_rate, 2), "forest_chains_ms": round(cx_chains, 3),
+                       "achieved_tflops": round(cx_flops, 3), "frac": round(cx_flops / PEAK_FP64_TFLOPS, 4),
+                       "max_rel_diff_vs_real": float(torch.max(torch.abs(e_cx - e) / torch.clamp(torch.abs(e), min=1.0))),
+                       "kernel": "k_chains_energy + k_leaf_energy (complex128, Gauss-form MFMA products)"}
+        batch.set_arith('auto')
+        batch.set_timing(False)
 
     # the same energies with the state built by exact CNOT bond doubling (no
     # SVD; the reverse engine's representation) instead of gates + SVDs
@@ -159,7 +189,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex128",
+        "dtype": "f64" if real else "complex128",
         "data": "synthetic",
         "config": {"workload": "spsa-40q: 40 qubits, 4x(RY layer + CX 39->38->...->0), chi<=16, zero cutoffs, "
                                "4000 Z-strings (weight 1-6) + constant, 256 SPSA vectors per step",
@@ -176,9 +206,11 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                      "traffic": measured_traffic(),
                      "traffic_basis": "HBM bytes per step (all generations), rocprofv3 PMC, copy-calibrated",
-                     "kernel": "k_chains_energy (all generations of one step) + k_leaf_energy",
+                     "kernel": ("k_rchains_energy (real arithmetic)" if real else "k_chains_energy")
+                               + " (all generations of one step) + leaf term-end kernel",
                      "algorithmic_flops_per_step": node_flops * B,
                      "executed_tflops": round(executed, 3)},
+        "complex_path": complex_leg,
     }
     if grad is not None:
         out["adjoint"] = grad

@@ -196,6 +196,8 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->n_spill_total = spill;
   B->d_inst = upload(B->owned, B->inst.data(), B->inst.size());
   B->arena = upload<double2>(B->owned, nullptr, site);
+  B->real = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->real_ok; });
+  if (B->real) B->rarena = upload<double>(B->owned, nullptr, site);
   B->envs = upload<double2>(B->owned, nullptr, env);
   B->spills = upload<double2>(B->owned, nullptr, spill);
   B->terms = upload<double2>(B->owned, nullptr, term);
@@ -303,6 +305,25 @@ void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
 void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
   const int ninst = (int)B.inst.size();
   B.mark(2, stream);
+  if (B.real && B.arith == 0) {
+    // real circuits, no Y words: the same forests in real arithmetic
+    hip_check(kl_pack_real(B.n_site_total, stream, B.arena, B.rarena), "k_pack_real launch");
+    for (int g = 0; g < B.max_gen; ++g) {
+      const int ni = B.n_items[g];
+      if (ni == 0) continue;
+      hip_check(kl_rchains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.leafs, B.spills,
+                           terms ? B.terms : nullptr, B.parts),
+                "k_rchains launch");
+    }
+    hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills,
+                             terms ? B.terms : nullptr, B.parts),
+              "k_rleaf_terms launch");
+    B.mark(3, stream);
+    if (terms) hip_check(kl_reduce(ninst, stream, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
+    else hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 1), "k_reduce launch");
+    B.mark(4, stream);
+    return;
+  }
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
@@ -546,6 +567,7 @@ int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
     info->flops_grad_executed = P.flops_grad;
     info->n_rotations = (int)P.rot_param.size();
     info->grad_ok = P.grad_error.empty() ? 1 : 0;
+    info->real_arith = P.real_ok ? 1 : 0;
   })
 }
 
@@ -701,6 +723,14 @@ int gvmps_batch_set_prep(gvmps_batch *b, int mode) {
       for (const Problem *P : B.probs)
         if (!P->grad_error.empty()) throw std::runtime_error(P->grad_error);
     B.prep_mode = mode;
+  })
+}
+
+int gvmps_batch_set_arith(gvmps_batch *b, int mode) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (mode != 0 && mode != 1) throw std::runtime_error("arithmetic mode must be 0 (automatic) or 1 (complex)");
+    B.arith = mode;
   })
 }
 

@@ -43,6 +43,10 @@ struct Batch {
   int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
   int order = 0;  // 0: instance-major items, 1: longest chains first
   int prep_mode = 0;  // 0: gates + SVDs (k_prep), 1: exact bond doubling (k_build_local)
+  // real-arithmetic energies: every problem has real gates and no Y words
+  bool real = false;
+  int arith = 0;              // 0: real path when `real`, 1: complex path always
+  double *rarena = nullptr;   // real parts of the site arena (real path)
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   // captured evaluate + sample sequence
   hipStream_t cap_stream = nullptr;
@@ -97,6 +101,13 @@ hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *item
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                             const double *parts, double *energies, int with_leaves);
 void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
+// kernel launchers (kernels_real.hip)
+hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena);
+hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+                      const ProbDev *probs, const double *rarena, double2 *leafs, double2 *spills, double2 *terms,
+                      double *parts);
+hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+                          const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 // kernel launchers (kernels_grad.hip)
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const double *params, double2 *garena, double2 *rotin, int *status);

@@ -676,6 +676,24 @@ void build_grad_meta(Problem &P) {
   }
 }
 
+// Every gate matrix of the tape is real: RY rotations (RX and RZ only as
+// constant identities), fixed single-site gates and two-site gates with zero
+// imaginary parts.  Applied to |0...0>, such a tape keeps the MPS real.
+bool real_circuit(const Problem &P) {
+  for (const ProgOp &op : P.prog) {
+    if (op.kind == OP_ROT) {
+      if (op.arg != 1 && !(op.param < 0 && op.offset == 0.0)) return false;
+    } else if (op.kind == OP_FIX1) {
+      for (int e = 0; e < 4; ++e)
+        if (P.fixes[(size_t)op.arg * 8 + 2 * e + 1] != 0.0) return false;
+    } else {
+      for (int e = 0; e < 16; ++e)
+        if (P.gates[(size_t)op.arg * 32 + 2 * e + 1] != 0.0) return false;
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int n_ops, int n_params,
@@ -710,6 +728,9 @@ std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int 
     if (!identity) P->active.push_back(k);
     P->site_words[k] = std::move(w);
   }
+  P->real_ok = real_circuit(*P) &&
+               std::none_of(P->site_words.begin(), P->site_words.end(),
+                            [](const std::string &w) { return w.find('Y') != std::string::npos; });
   build_chains(*P);
   build_grad_meta(*P);
   return P;

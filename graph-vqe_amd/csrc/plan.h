@@ -33,6 +33,7 @@ struct Problem {
   std::vector<double> coeffs;            // 2*nterms
   std::vector<std::string> site_words;   // site order
   std::vector<int> active;               // non-identity terms
+  bool real_ok = false;                  // real gate matrices and no Y in any word: real-arithmetic energies
   // prefix forest, chain schedule
   std::vector<ChainDev> chains;
   std::vector<StepDev> steps;

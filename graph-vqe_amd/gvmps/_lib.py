@@ -27,7 +27,8 @@ class ProblemInfo(ctypes.Structure):
         'n_sched_steps', 'n_right_nodes')] + [
         ('flops_energy_executed', ctypes.c_double), ('flops_energy_nodes', ctypes.c_double),
         ('flops_svd_nominal', ctypes.c_double), ('flops_grad_executed', ctypes.c_double),
-        ('n_rotations', ctypes.c_int32), ('grad_ok', ctypes.c_int32)]
+        ('n_rotations', ctypes.c_int32), ('grad_ok', ctypes.c_int32),
+        ('real_arith', ctypes.c_int32)]
 
 
 # (name, argtypes); every function returns int status except where noted
@@ -55,6 +56,7 @@ _PROTOS = {
                                ctypes.c_void_p],
     'gvmps_batch_set_launch': [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
     'gvmps_batch_set_prep': [ctypes.c_void_p, ctypes.c_int],
+    'gvmps_batch_set_arith': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_set_checkpoint': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_evaluate': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],

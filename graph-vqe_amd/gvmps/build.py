@@ -14,7 +14,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = ROOT / 'csrc'
 LIB = PKG / 'libgvmps.so'
-HIP_SOURCES = ['kernels_prep.hip', 'kernels_energy.hip', 'kernels_sample.hip', 'kernels_grad.hip', 'api.hip',
+HIP_SOURCES = ['kernels_prep.hip', 'kernels_energy.hip', 'kernels_real.hip', 'kernels_sample.hip', 'kernels_grad.hip',
+               'api.hip',
                'kernels_gnn.hip', 'gnn_api.hip']
 CPP_SOURCES = ['plan.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')

@@ -171,6 +171,13 @@ class Batch:
         check(lib().gvmps_batch_set_prep(self._h, m))
         self.prep = mode
 
+    def set_arith(self, mode):
+        """'auto' (real-arithmetic forests when every problem has real gates
+        and no Y words) or 'complex' (complex kernels always)."""
+        m = {'auto': 0, 'complex': 1}[mode]
+        check(lib().gvmps_batch_set_arith(self._h, m))
+        self.arith = mode
+
     def _eval_buffers(self, shots):
         t = self.torch
         dev = f'cuda:{self.device}'

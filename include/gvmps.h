@@ -54,6 +54,7 @@ typedef struct gvmps_problem_info {
   double flops_grad_executed;      /* FP64 flops of one reverse-gradient pass (forest part) */
   int32_t n_rotations;             /* parameterised + constant rotations of the tape */
   int32_t grad_ok;                 /* 1: the tape has a bond-doubling reverse gradient */
+  int32_t real_arith;              /* 1: real gate matrices and no Y words (real-arithmetic energies) */
 } gvmps_problem_info;
 
 /* Last error message of the calling thread ("" if none). */
@@ -158,6 +159,15 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order);
  * equal to mode 0 within rounding.  read_sites then returns doubling tensors
  * and no bond profile. */
 int gvmps_batch_set_prep(gvmps_batch *b, int mode);
+
+/* Arithmetic of the energy forests.  When every problem of the batch has real
+ * gate matrices and no Y in its Pauli words (real_arith in the problem info),
+ * mode 0 (default) runs the forests in real arithmetic: the state is exactly
+ * real, so each complex product reduces to one real product.  Mode 1 forces
+ * the complex kernels.  Energies agree to rounding.  No reference counterpart
+ * (the plugin is complex throughout, sharded_mps.cuh:614-687); a switch for
+ * comparison and tests. */
+int gvmps_batch_set_arith(gvmps_batch *b, int mode);
 
 /* Energies plus, for shots > 0, `shots` samples per instance from ONE state
  * preparation.  Replaces an observe() followed by a sample() on the same

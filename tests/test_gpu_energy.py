@@ -71,6 +71,48 @@ def test_random_circuits_xyz_words_vs_dense(cuda, seed):
         assert rel_close(e[i], d.energy(words, coeffs))
 
 
+@pytest.mark.parametrize('seed', range(3))
+def test_real_circuits_real_path_vs_complex_and_dense(cuda, seed):
+    """Real gates on |0..0> and I/X/Z words run the real-arithmetic forests;
+    they must agree with the complex kernels and the dense statevector."""
+    r = np.random.default_rng(300 + seed)
+    n = int(r.integers(4, 10))
+    ops, P = [], 0
+    for _ in range(70):
+        kind = r.integers(0, 4)
+        q = int(r.integers(0, n))
+        if kind == 0:
+            ops.append(('ry', (q,), P))
+            P += 1
+        elif kind == 1:
+            ops.append((str(r.choice(['h', 'x', 'z'])), (q,), None))
+        else:
+            t = int(r.integers(0, n - 1))
+            ops.append((str(r.choice(['cx', 'cx', 'swap'])), (q, t + 1 if t >= q else t), None))
+    words = [''.join(r.choice(list('IXZ'), n)) for _ in range(50)] + ['I' * n]
+    coeffs = r.normal(size=len(words))
+    X = r.uniform(-np.pi, np.pi, (4, P))
+    print(f'case: real circuit seed={seed} n={n} P={P}', flush=True)
+    p = Problem(n, ops, P, words, coeffs, max_bond=16, abs_cut=0, rel_cut=0)
+    assert p.info()['real_arith'] == 1
+    b = Batch([p], [4])
+    e_re, t_re = b.energies(X, terms=True)
+    e_plain = b.energies(X).cpu().numpy()
+    b.set_arith('complex')
+    e_cx, t_cx = b.energies(X, terms=True)
+    e_re, e_cx = e_re.cpu().numpy(), e_cx.cpu().numpy()
+    t_re, t_cx = t_re.cpu().numpy().reshape(4, -1), t_cx.cpu().numpy().reshape(4, -1)
+    assert np.all(t_re.imag == 0.0)
+    np.testing.assert_allclose(t_re.real, t_cx.real, atol=1e-12)
+    assert rel_close(e_re, e_cx, 1e-12) and rel_close(e_plain, e_cx, 1e-12)
+    for i in range(4):
+        d = dense.Dense(n).run(tape_with_values(ops, X[i]))
+        exact = np.array([d.expect(wd) for wd in words])
+        np.testing.assert_allclose(t_re[i].real, exact.real, atol=1e-11)
+        assert rel_close(e_re[i], d.energy(words, coeffs))
+        assert rel_close(e_plain[i], d.energy(words, coeffs))
+
+
 def test_six_entanglement_patterns_vs_dense(cuda):
     n = 6
     labels = [("IIIIIZ", 0.3), ("IIIIZZ", -0.7), ("IIZIII", 0.2), ("IIIIIX", 0.15), ("IIIYII", -0.1),

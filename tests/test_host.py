@@ -56,6 +56,18 @@ def test_reverse_gradient_support_rules():
     assert Problem(4, many[:4], 0, words, coeffs, max_bond=16).info()['grad_ok'] == 1
 
 
+def test_real_arithmetic_rules():
+    """real_arith: every gate matrix real (RY, H, X, Z, CX, SWAP; RX/RZ only as
+    constant identities) and no Y in any word."""
+    words, coeffs = ['ZXIZ', 'IIII'], [1.0, 0.5]
+    real_ops = [('ry', (0,), 0), ('h', (1,), None), ('cx', (1, 0), None), ('swap', (2, 3), None),
+                ('x', (2,), None), ('z', (3,), None), ('rz', (1,), 0.0)]
+    assert Problem(4, real_ops, 1, words, coeffs).info()['real_arith'] == 1
+    for extra in [('rx', (0,), 0), ('rz', (2,), 0), ('s', (1,), None), ('y', (0,), None), ('rx', (3,), 0.3)]:
+        assert Problem(4, real_ops + [extra], 1, words, coeffs).info()['real_arith'] == 0, extra
+    assert Problem(4, real_ops, 1, ['ZYII'], [1.0]).info()['real_arith'] == 0
+
+
 def test_planner_rejects_bad_input():
     with pytest.raises(RuntimeError, match='>= 2 qubits'):
         Problem(1, [], 0, ['Z'], [1.0])
