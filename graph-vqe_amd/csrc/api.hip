@@ -295,7 +295,8 @@ void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
     hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.arena, nullptr, B.status),
               "k_build_local launch");
   else
-    hip_check(kl_prep(ninst, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status, B.stats),
+    hip_check(kl_prep(ninst, B.real && B.arith == 0, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status,
+                      B.stats),
               "k_prep launch");
   B.mark(1, stream);
 }

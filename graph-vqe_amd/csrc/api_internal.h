@@ -86,8 +86,8 @@ void launch_prepare(Batch &B, const double *params, hipStream_t stream);
 void launch_energy_tail(Batch &B, hipStream_t stream, bool terms);
 
 // kernel launchers (kernels_energy.hip)
-hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, const double *params,
-                   double2 *arena, int *bonds, int *status, int *stats);
+hipError_t kl_prep(int ninst, bool real, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                   const double *params, double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
 hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                      const double2 *arena, double2 *leafs, double2 *spills, double2 *terms);

@@ -26,6 +26,8 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 // registers (~240 VGPRs); capping them at 256 registers gives two waves per
 // SIMD, so one wave's memory latency hides behind the other's MFMAs.
 #define GV_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+// The real-arithmetic chains need about 120 registers: four waves per SIMD.
+#define GV_FOUR_WAVES __attribute__((amdgpu_waves_per_eu(4, 4)))
 
 // Index of this wavefront in its workgroup as a wave-uniform value.  Without
 // readfirstlane the compiler treats threadIdx.x >> 6 as per-lane, so every

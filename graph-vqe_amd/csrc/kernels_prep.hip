@@ -154,7 +154,8 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k)
 // One Householder step j of the pivoted QR, j in [R0, R0 + 4).  Rows < R0 are
 // final and not touched; rows R0..j-1 of the block are masked.  Returns false
 // when the Theta block is exhausted (every remaining column norm is zero).
-template <int R0>
+// REAL: the panel is real (imaginary parts zero and not carried).
+template <bool REAL, int R0>
 __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, int nc, int lane, double &piv0) {
   // squared norm of rows >= j of this lane's column (four independent sums)
   double n4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -162,7 +163,8 @@ __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, 
   for (int r = R0; r < 32; ++r) {
     double2 c = col[r];
     if (r < R0 + 4 && r < j) c = make_double2(0.0, 0.0);
-    n4[r & 3] = fma(c.x, c.x, fma(c.y, c.y, n4[r & 3]));
+    if constexpr (REAL) n4[r & 3] = fma(c.x, c.x, n4[r & 3]);
+    else n4[r & 3] = fma(c.x, c.x, fma(c.y, c.y, n4[r & 3]));
   }
   const double nrm = (n4[0] + n4[1]) + (n4[2] + n4[3]);
   // pivot: the largest norm, ties to the lowest column.  Norms are >= 0, so
@@ -179,7 +181,7 @@ __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, 
 #pragma unroll
   for (int r = R0 + 1; r < R0 + 4; ++r)
     if (r == j) xl = col[r];
-  const double2 xj = make_double2(readlane_f64(xl.x, bi), readlane_f64(xl.y, bi));
+  const double2 xj = make_double2(readlane_f64(xl.x, bi), REAL ? 0.0 : readlane_f64(xl.y, bi));
   // v = x + e |x| e_j, e = x_j / |x_j|;  tau = 2 / (v^H v) = 1 / (|x| (|x| + |x_j|))
   const double alpha = sqrt(a2);
   const double axj = sqrt(xj.x * xj.x + xj.y * xj.y);
@@ -201,16 +203,24 @@ __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, 
 #pragma unroll
   for (int r = R0; r < 32; ++r) {
     const double2 v = vb[r], c = col[r];
-    dr[r & 3] = fma(v.x, c.x, fma(v.y, c.y, dr[r & 3]));  // conj(v) c
-    di[r & 3] = fma(v.x, c.y, fma(-v.y, c.x, di[r & 3]));
+    if constexpr (REAL) {
+      dr[r & 3] = fma(v.x, c.x, dr[r & 3]);
+    } else {
+      dr[r & 3] = fma(v.x, c.x, fma(v.y, c.y, dr[r & 3]));  // conj(v) c
+      di[r & 3] = fma(v.x, c.y, fma(-v.y, c.x, di[r & 3]));
+    }
   }
   const double dx = tau * ((dr[0] + dr[1]) + (dr[2] + dr[3]));
   const double dy = tau * ((di[0] + di[1]) + (di[2] + di[3]));
 #pragma unroll
   for (int r = R0; r < 32; ++r) {
     const double2 w = vb[r];
-    col[r].x -= w.x * dx - w.y * dy;
-    col[r].y -= w.x * dy + w.y * dx;
+    if constexpr (REAL) {
+      col[r].x -= w.x * dx;
+    } else {
+      col[r].x -= w.x * dx - w.y * dy;
+      col[r].y -= w.x * dy + w.y * dx;
+    }
   }
   wave_sync();
   return true;
@@ -222,21 +232,26 @@ __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, 
 // Lane c keeps panel column c in registers (rows >= m are zero and stay
 // zero: the reflectors vanish there); the reflector of each step is broadcast
 // through `vb`.  Returns the squared Frobenius norm of the rows >= `steps` of
-// Q^H Theta (the trailing block R22).  steps <= 16.
+// Q^H Theta (the trailing block R22).  steps <= 16.  REAL: a real panel
+// (imaginary parts zero), real arithmetic.
+template <bool REAL>
 __device__ double wave_pivoted_qr(double2 (*W)[PW], double2 *vb, int m, int nc, int width, int steps, int lane,
                                   double &piv0) {
   piv0 = 0.0;
   double2 col[32];
   const bool own = lane < width;
 #pragma unroll
-  for (int r = 0; r < 32; ++r) col[r] = (own && r < m) ? W[r][lane] : make_double2(0.0, 0.0);
+  for (int r = 0; r < 32; ++r) {
+    col[r] = (own && r < m) ? W[r][lane] : make_double2(0.0, 0.0);
+    if constexpr (REAL) col[r].y = 0.0;
+  }
   for (int j = 0; j < steps; ++j) {
     bool more;
     switch (j >> 2) {
-      case 0: more = qr_step<0>(col, vb, j, nc, lane, piv0); break;
-      case 1: more = qr_step<4>(col, vb, j, nc, lane, piv0); break;
-      case 2: more = qr_step<8>(col, vb, j, nc, lane, piv0); break;
-      default: more = qr_step<12>(col, vb, j, nc, lane, piv0); break;
+      case 0: more = qr_step<REAL, 0>(col, vb, j, nc, lane, piv0); break;
+      case 1: more = qr_step<REAL, 4>(col, vb, j, nc, lane, piv0); break;
+      case 2: more = qr_step<REAL, 8>(col, vb, j, nc, lane, piv0); break;
+      default: more = qr_step<REAL, 12>(col, vb, j, nc, lane, piv0); break;
     }
     if (!more) break;
   }
@@ -302,6 +317,10 @@ __device__ int wave_jacobi(double2 (*W)[PW], int rows, int nc, int width, int la
   return sweep + 1;
 }
 
+// REAL: the circuit's gate matrices are real, so the sites, the pending
+// single-site matrices and Theta are real; the Theta products take one MFMA
+// product instead of four and the QR runs in real arithmetic.
+template <bool REAL>
 __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const GateDev gd, const double *x,
                          double2 *sites, int *status) {
   const unsigned long long t0 = clk();
@@ -336,9 +355,11 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         zr = mfma(ar[i][t], br[j][t], zr);
-        zr = mfma(-ai[i][t], bi[j][t], zr);
-        zi = mfma(ar[i][t], bi[j][t], zi);
-        zi = mfma(ai[i][t], br[j][t], zi);
+        if constexpr (!REAL) {
+          zr = mfma(-ai[i][t], bi[j][t], zr);
+          zi = mfma(ar[i][t], bi[j][t], zi);
+          zi = mfma(ai[i][t], br[j][t], zi);
+        }
       }
       cr[i][j] = zr;
       ci[i][j] = zi;
@@ -363,7 +384,8 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
               const double2 gv = G[(I * 2 + J) * 4 + (i * 2 + j)];
-              val = cadd(val, cmul(gv, make_double2(cr[i][j][v], ci[i][j][v])));
+              if constexpr (REAL) val.x += gv.x * cr[i][j][v];
+              else val = cadd(val, cmul(gv, make_double2(cr[i][j][v], ci[i][j][v])));
             }
           W[l * 2 + I][J * Dr + r] = val;
           frob += val.x * val.x + val.y * val.y;
@@ -386,7 +408,7 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
   const bool zero_cut = (P.abs_cut == 0.0 && P.rel_cut == 0.0);
   const int rp = min(min(count, GV_CHI), P.max_bond + (P.max_bond & 1));
   double piv0;
-  const double trail = wave_pivoted_qr(W, L.vb[w], m, nc, nc + m, rp, lane, piv0);
+  const double trail = wave_pivoted_qr<REAL>(W, L.vb[w], m, nc, nc + m, rp, lane, piv0);
   const double s0_lower = sqrt(piv0);  // largest column norm <= sigma_0
   const double lim = zero_cut ? 64.0 * 2.2204460492503131e-16 * count * s0_lower
                               : fmax(P.abs_cut, P.rel_cut * s0_lower);
@@ -484,9 +506,11 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
   }
 }
 
-extern "C" __global__ void __launch_bounds__(64 * GV_WAVES)
-k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
-       double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status, int *__restrict__ stats) {
+template <bool REAL>
+__device__ __forceinline__ void prep_body(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                          const double *__restrict__ params, double2 *__restrict__ arena,
+                                          int *__restrict__ bonds_out, int *__restrict__ status,
+                                          int *__restrict__ stats) {
   __shared__ PrepLds L;
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   const InstDev inst = uload(insts + blockIdx.x);
@@ -508,7 +532,7 @@ k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, con
 
   for (int row = 0; row < P.nsched; ++row) {
     const int gi = uload(P.sched + row * GV_WAVES + w);
-    if (gi >= 0) run_gate(L, w, lane, P, uload(P.g2 + gi), x, sites, st);
+    if (gi >= 0) run_gate<REAL>(L, w, lane, P, uload(P.g2 + gi), x, sites, st);
     __syncthreads();
   }
   // single-site ops left pending after the last two-site gate on each site
@@ -534,12 +558,30 @@ k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, con
   }
 }
 
+extern "C" __global__ void __launch_bounds__(64 * GV_WAVES)
+k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
+       double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status, int *__restrict__ stats) {
+  prep_body<false>(insts, probs, params, arena, bonds_out, status, stats);
+}
+
+// Real circuits (every gate matrix real): the same preparation in real arithmetic.
+extern "C" __global__ void __launch_bounds__(64 * GV_WAVES)
+k_prep_real(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
+            double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status,
+            int *__restrict__ stats) {
+  prep_body<true>(insts, probs, params, arena, bonds_out, status, stats);
+}
+
 namespace gv {
 
-hipError_t kl_prep(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, const double *params,
-                   double2 *arena, int *bonds, int *status, int *stats) {
-  hipLaunchKernelGGL(k_prep, dim3(ninst), dim3(64 * GV_WAVES), 0, s, insts, probs, params, arena, bonds, status,
-                     stats);
+hipError_t kl_prep(int ninst, bool real, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                   const double *params, double2 *arena, int *bonds, int *status, int *stats) {
+  if (real)
+    hipLaunchKernelGGL(k_prep_real, dim3(ninst), dim3(64 * GV_WAVES), 0, s, insts, probs, params, arena, bonds,
+                       status, stats);
+  else
+    hipLaunchKernelGGL(k_prep, dim3(ninst), dim3(64 * GV_WAVES), 0, s, insts, probs, params, arena, bonds, status,
+                       stats);
   return hipGetLastError();
 }
 

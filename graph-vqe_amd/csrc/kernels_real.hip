@@ -218,14 +218,14 @@ k_pack_real(const double2 *__restrict__ arena, double *__restrict__ rarena, long
     rarena[i] = arena[i].x;
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
           const ProbDev *__restrict__ probs, const double *__restrict__ rarena, double2 *__restrict__ leafs,
           double2 *__restrict__ spills, double2 *__restrict__ terms) {
   rchains_body<true>(items, nitems, insts, probs, rarena, leafs, spills, terms, nullptr);
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                  const ProbDev *__restrict__ probs, const double *__restrict__ rarena, double2 *__restrict__ leafs,
                  double2 *__restrict__ spills, double *__restrict__ parts) {
