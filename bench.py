@@ -343,7 +343,10 @@ def bench_grads(args, prob, info, x, B):
     names = ["bond_doubling_build", "forward_chains", "energy_reduce", "backward_chains", "site_reduce",
              "local_backward"]
     fwd_ms, bwd_ms = phase_ms[1] / c, phase_ms[3] / c
-    flops = info['flops_grad_executed'] * B
+    # complex count: 4 real products per complex product; the real path
+    # (real tape, no Y words) executes one
+    real = bool(info['real_arith'])
+    flops = info['flops_grad_executed'] * B / (4.0 if real else 1.0)
     achieved = flops / ((fwd_ms + bwd_ms) * 1e-3) / 1e12
     return {
         "value": round(rate, 2), "unit": "adjoint grads/s", "steps": args.grad_steps,
@@ -352,6 +355,7 @@ def bench_grads(args, prob, info, x, B):
         "batched_vs_plain": plain_rate and round(rate / plain_rate, 2),
         "plain_matches_batched_bitwise": same,
         "phase_ms_per_step": {n: round(phase_ms[i] / c, 3) for i, n in enumerate(names)},
+        "arithmetic": "real" if real else "complex",
         "chains_tflops": round(achieved, 3),
         "chains_frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 4),
     }

@@ -448,6 +448,37 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   B.mark(0, stream);
   hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.garena, B.rotin, nullptr),
             "k_build_local launch");
+  if (B.real && B.arith == 0) {
+    // real tape, no Y words: the forests and their adjoints in real arithmetic
+    hip_check(kl_pack_real(B.n_site_total, stream, B.garena, B.rarena), "k_pack_real launch");
+    B.mark(1, stream);
+    for (int g = 0; g < B.max_gen; ++g) {
+      const int ni = B.n_items[g];
+      if (ni == 0) continue;
+      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.spills, B.parts,
+                                B.estore, B.gleafs),
+                "k_rchains_gfwd launch");
+    }
+    B.mark(2, stream);
+    hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 0), "k_reduce launch");
+    B.mark(3, stream);
+    for (int g = B.max_gen - 1; g >= 0; --g) {
+      const int ni = B.n_items[g];
+      if (ni == 0) continue;
+      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.spills, B.adjspill,
+                               B.estore, B.gleafs, B.aslot),
+                "k_rchains_bwd launch");
+    }
+    B.mark(4, stream);
+    hip_check(kl_rsite_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_rsite_reduce launch");
+    B.mark(5, stream);
+    hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad),
+              "k_site_bwd launch");
+    hip_check(kl_param_grad(ninst, B.max_params, stream, B.d_inst, B.d_probs, B.rgrad, B.grads),
+              "k_param_grad launch");
+    B.mark(6, stream);
+    return;
+  }
   B.mark(1, stream);
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];

@@ -108,6 +108,14 @@ hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, con
                       double *parts);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
+hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+                           const ProbDev *probs, const double *rgarena, double2 *spills, double *parts, double2 *estore,
+                           double2 *gleafs);
+hipError_t kl_rchains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+                          const ProbDev *probs, const double *rgarena, const double2 *spills, double2 *adjspill,
+                          const double2 *estore, const double2 *gleafs, double2 *aslot);
+hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                           const double2 *aslot, double2 *abar);
 // kernel launchers (kernels_grad.hip)
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const double *params, double2 *garena, double2 *rotin, int *status);

@@ -208,7 +208,293 @@ __device__ __forceinline__ void rleaf_body(const int2 *__restrict__ items, int n
   }
 }
 
+__device__ __forceinline__ int sidx(int l, int i, int r) { return (l * 2 + i) * 16 + r; }
+
+// Input environment of the step after `step` in its chain (the continuing
+// child), with the products and combination of k_rchains_gfwd: bit for bit
+// the environment the forward pass had.
+__device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, int step, bool transposed, int lane,
+                                         REnv &e) {
+  const StepDev sd = uload(P.steps + step);
+  RFrag f;
+  rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+  int kind = CK_I;
+  for (int c = 0; c < sd.nchild; ++c) {
+    const ChildDev cd = uload(P.children + sd.child0 + c);
+    if (cd.action == -2) kind = cd.kind;
+  }
+  const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
+  const bool xk = kind == CK_X;
+  const d4 ya = rprod_y(xk ? x1 : x0, f, 0);
+  const d4 yb = rprod_y(xk ? x0 : x1, f, 1);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) e.v[v] = kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
+}
+
 }  // namespace
+
+// ------------------------------------------------ reverse gradient, real
+// The bond-doubling state of a real tape (RY, CX, ...) is real, and so are
+// the forest environments of I/X/Z words and their adjoints (the adjoint of a
+// real symmetric environment is real symmetric: conj and transpose drop out
+// of the complex formulas).  Same slots as kernels_grad.hip; environments use
+// the first 2 KiB of each 4 KiB slot, site adjoints the first 4 KiB of each
+// 8 KiB step slot.
+
+// Forward: energy chains keeping the checkpointed step environments and the
+// prefix leaves the backward pass reads (k_chains_gfwd in real arithmetic).
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
+k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena, double2 *__restrict__ spills,
+               double *__restrict__ parts, double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
+  const int lane = threadIdx.x & 63;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+  const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
+  const int w = logical * (int)(blockDim.x >> 6) + wave_id();
+  if (w >= nitems) return;
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
+  const double *sites = rgarena + inst.site_off;
+  double2 *spill = spills + inst.spill_off;
+  double2 *es = estore + inst.ck_off * GV_ENV;
+  double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
+  REnv e;
+  if (ch.src >= 0) renv_load(rslot(spill, ch.src), lane, e);
+  else renv_unit(lane, e);
+  const bool transposed = ch.dir != 0;
+  double acc = 0.0;
+  StepDev sd0 = uload(P.steps + ch.step0);
+  RFrag fa, fb;
+  rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa, transposed);
+  auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
+    if (st + 1 < ch.nsteps) {
+      nsd = uload(P.steps + ch.step0 + st + 1);
+      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, transposed);
+    }
+    const int cs = uload(P.ck_slot + ch.step0 + st);
+    if (cs >= 0) renv_store(rslot(es, cs), lane, e);
+    const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
+    REnv cont;
+#pragma nounroll
+    for (int pass = 0; pass < 2; ++pass) {
+      if (!(sd.need & (1 << pass))) continue;
+      const d4 ya = rprod_y(pass ? x1 : x0, f, 0);
+      const d4 yb = rprod_y(pass ? x0 : x1, f, 1);
+      for (int c = 0; c < sd.nchild; ++c) {
+        const ChildDev cd = uload(P.children + sd.child0 + c);
+        if ((cd.kind == CK_X) != (pass == 1)) continue;
+        REnv ce;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
+        if (cd.store >= 0) renv_store(rslot(spill, cd.store), lane, ce);
+        if (cd.gleaf >= 0) renv_store(rslot(gl, cd.gleaf), lane, ce);
+        for (int t = 0; t < cd.nterm; ++t) {
+          const TermEnd te = uload(P.term_list + cd.term0 + t);
+          REnv r;
+          if (te.rslot >= 0) renv_load(rslot(spill, te.rslot), lane, r);
+          else renv_unit(lane, r);
+          double sr = 0.0;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sr += ce.v[v] * r.v[v];
+          acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c_k) <P_k>
+        }
+        if (cd.action == -2) cont = ce;
+        else if (cd.action >= 0) renv_store(rslot(spill, cd.action), lane, ce);
+      }
+    }
+    e = cont;
+  };
+  for (int st = 0; st < ch.nsteps; st += 2) {
+    StepDev sd1;
+    run_step(st, sd0, sd1, fa, fb);
+    if (st + 1 >= ch.nsteps) break;
+    run_step(st + 1, sd1, sd0, fb, fa);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) parts[inst.part_off + it.y] = acc;
+}
+
+// Backward (k_chains_bwd in real arithmetic).  For a step with input E and
+// site matrices M_i: Ybar00 = I+Z, Ybar11 = I-Z, Ybar01 = Ybar10 = X (child
+// adjoints), G_i = sum_j M_j Ybar_ji, Ebar = sum_i G_i M_i^T, Mbar_i = 2 E G_i.
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
+k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+              const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+              const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
+              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
+  __shared__ double tr[4][16][17];
+  const int lane = threadIdx.x & 63;
+  const int wv = wave_id();
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+  const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
+  const int w = logical * (int)(blockDim.x >> 6) + wv;
+  if (w >= nitems) return;
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
+  const double *sites = rgarena + inst.site_off;
+  const double2 *spill = spills + inst.spill_off;
+  double2 *aspill = adjspill + inst.spill_off;
+  const double2 *es = estore + inst.ck_off * GV_ENV;
+  const double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
+  double2 *as = aslot + inst.step_off * GV_SLOT;
+  const bool transposed = ch.dir != 0;
+  const int a16 = lane & 15, g4 = lane >> 4;
+
+  REnv eb;  // adjoint of the environment continuing the chain
+#pragma unroll
+  for (int t = 0; t < 4; ++t) eb.v[t] = 0.0;
+  for (int st = ch.nsteps - 1; st >= 0; --st) {
+    const int step = ch.step0 + st;
+    const StepDev sd = uload(P.steps + step);
+    REnv E;
+    const int cs = uload(P.ck_slot + step);
+    if (cs >= 0) {
+      renv_load(rslot(es, cs), lane, E);
+    } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
+      int s0 = st;
+      while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
+      renv_load(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
+      for (int q = s0; q < st; ++q) radvance(P, sites, ch.step0 + q, transposed, lane, E);
+    }
+    RFrag fa;  // M_i in the A-operand form
+    rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, fa, !transposed);
+    REnv y00, y11, y01;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) y00.v[t] = y11.v[t] = y01.v[t] = 0.0;
+    for (int c = 0; c < sd.nchild; ++c) {
+      const ChildDev cd = uload(P.children + sd.child0 + c);
+      REnv cb;
+      if (cd.action == -2) cb = eb;
+      else if (cd.action >= 0) renv_load(rslot(aspill, cd.action), lane, cb);
+      else {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cb.v[t] = 0.0;
+      }
+      // term ends (prefix forest): Cbar += Re(c_k) Rt_k
+      for (int t = 0; t < cd.nterm; ++t) {
+        const TermEnd te = uload(P.term_list + cd.term0 + t);
+        const double ck = uload(P.coeffs + 2 * te.term);
+        if (te.rslot >= 0) {
+          REnv r;
+          renv_load(rslot(spill, te.rslot), lane, r);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) cb.v[v] += ck * r.v[v];
+        } else if (lane == 0) {
+          cb.v[0] += ck;  // unit right environment
+        }
+      }
+      // suffix-forest slot read by prefix term ends: Cbar += Re(c_k) E_left(k)
+      if (cd.store >= 0) {
+        const int q1 = uload(P.seed_begin + cd.store + 1);
+        for (int q = uload(P.seed_begin + cd.store); q < q1; ++q) {
+          const SeedTerm sdt = uload(P.seed_terms + q);
+          REnv le;
+          renv_load(rslot(gl, sdt.gleaf), lane, le);
+          const double ck = uload(P.coeffs + 2 * sdt.term);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) cb.v[v] += ck * le.v[v];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (cd.kind == CK_I) { y00.v[t] += cb.v[t]; y11.v[t] += cb.v[t]; }
+        else if (cd.kind == CK_Z) { y00.v[t] += cb.v[t]; y11.v[t] -= cb.v[t]; }
+        else y01.v[t] += cb.v[t];  // X: Ybar01 = Ybar10
+      }
+    }
+    // G_i (as the next products' A operand): G_0 = Ybar00 M_0 + Ybar01 M_1, G_1 = Ybar11 M_1 + Ybar10 M_0
+    d4 g0 = {0.0, 0.0, 0.0, 0.0}, g1 = {0.0, 0.0, 0.0, 0.0};
+    if (sd.need & 1) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        g0 = mfma(y00.v[t], fa.v[0][t], g0);
+        g1 = mfma(y11.v[t], fa.v[1][t], g1);
+      }
+    }
+    if (sd.need & 2) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        g0 = mfma(y01.v[t], fa.v[1][t], g0);
+        g1 = mfma(y01.v[t], fa.v[0][t], g1);
+      }
+    }
+    // Ebar = sum_i G_i M_i^T
+    d4 er = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      er = mfma(g0[t], fa.v[0][t], er);
+      er = mfma(g1[t], fa.v[1][t], er);
+    }
+    // Mbar_i = 2 E G_i: G_i to the B-operand layout through LDS
+    double *slot = (double *)(as + (size_t)step * GV_SLOT);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const d4 &g = i ? g1 : g0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) tr[wv][a16][kdx(g4, t)] = g[t];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double d[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) d[v] = 2.0 * tr[wv][kdx(g4, v)][a16];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      d4 m = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) m = mfma(E.v[t], d[t], m);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = kdx(g4, v);
+        slot[transposed ? sidx(a16, i, row) : sidx(row, i, a16)] = m[v];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) eb.v[t] = er[t];
+  }
+  if (ch.src >= 0) renv_store(rslot(aspill, ch.src), lane, eb);
+}
+
+// Per (instance, site): sum of its steps' real adjoints in step order, written
+// as a complex slot (imaginary parts zero) for k_site_bwd.
+extern "C" __global__ void __launch_bounds__(256)
+k_rsite_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+               const double2 *__restrict__ aslot, double2 *__restrict__ abar) {
+  const InstDev inst = uload(insts + blockIdx.y);
+  const int s = blockIdx.x;
+  if (s >= inst.n) return;
+  const ProbDev P = uload(probs + inst.prob);
+  const int t = threadIdx.x;
+  const double2 *as = aslot + inst.step_off * GV_SLOT;
+  double s0 = 0.0, s1 = 0.0;
+  const int q1 = uload(P.site_steps_begin + s + 1);
+  int q = uload(P.site_steps_begin + s);
+  for (; q + 8 <= q1; q += 8) {
+    double2 a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = as[(size_t)uload(P.site_steps + q + u) * GV_SLOT + t];  // doubles 2t, 2t+1
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 += a[u].x;
+      s1 += a[u].y;
+    }
+  }
+  for (; q < q1; ++q) {
+    const double2 a = as[(size_t)uload(P.site_steps + q) * GV_SLOT + t];
+    s0 += a.x;
+    s1 += a.y;
+  }
+  double2 *dst = abar + inst.site_off + (size_t)s * GV_SLOT;
+  dst[2 * t] = make_double2(s0, 0.0);
+  dst[2 * t + 1] = make_double2(s1, 0.0);
+}
 
 // Real parts of the prepared site arena (imaginary parts are exactly zero on
 // this path).
@@ -263,6 +549,28 @@ hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, con
   else
     hipLaunchKernelGGL(k_rchains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts,
                        probs, rarena, leafs, spills, parts);
+  return hipGetLastError();
+}
+
+hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+                           const ProbDev *probs, const double *rgarena, double2 *spills, double *parts, double2 *estore,
+                           double2 *gleafs) {
+  hipLaunchKernelGGL(k_rchains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
+                     rgarena, spills, parts, estore, gleafs);
+  return hipGetLastError();
+}
+
+hipError_t kl_rchains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+                          const ProbDev *probs, const double *rgarena, const double2 *spills, double2 *adjspill,
+                          const double2 *estore, const double2 *gleafs, double2 *aslot) {
+  hipLaunchKernelGGL(k_rchains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
+                     rgarena, spills, adjspill, estore, gleafs, aslot);
+  return hipGetLastError();
+}
+
+hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                           const double2 *aslot, double2 *abar) {
+  hipLaunchKernelGGL(k_rsite_reduce, dim3(max_n, ninst), dim3(256), 0, s, insts, probs, aslot, abar);
   return hipGetLastError();
 }
 

@@ -160,13 +160,15 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order);
  * and no bond profile. */
 int gvmps_batch_set_prep(gvmps_batch *b, int mode);
 
-/* Arithmetic of the energy forests.  When every problem of the batch has real
- * gate matrices and no Y in its Pauli words (real_arith in the problem info),
- * mode 0 (default) runs the forests in real arithmetic: the state is exactly
- * real, so each complex product reduces to one real product.  Mode 1 forces
- * the complex kernels.  Energies agree to rounding.  No reference counterpart
- * (the plugin is complex throughout, sharded_mps.cuh:614-687); a switch for
- * comparison and tests. */
+/* Arithmetic of state preparation, energy forests and reverse gradients.
+ * When every problem of the batch has real gate matrices and no Y in its
+ * Pauli words (real_arith in the problem info), mode 0 (default) runs them in
+ * real arithmetic: the state is exactly real, so each complex product reduces
+ * to one real product.  A batch that mixes such problems with others runs
+ * all of its instances in complex arithmetic.  Mode 1 forces the complex
+ * kernels.  Results of the two agree to rounding; bitwise equality between a
+ * batch and one-at-a-time calls holds for the same arithmetic.  No reference
+ * counterpart (the plugin is complex throughout, sharded_mps.cuh:614-687). */
 int gvmps_batch_set_arith(gvmps_batch *b, int mode);
 
 /* Energies plus, for shots > 0, `shots` samples per instance from ONE state

@@ -105,6 +105,63 @@ def test_random_tapes_xyz_words(cuda, seed):
         np.testing.assert_allclose(g[i], go, atol=1e-8, rtol=0)
 
 
+@pytest.mark.parametrize('seed', range(3))
+def test_real_tapes_real_path(cuda, seed):
+    """Real tapes (RY, H, X, Z, nearest-neighbour CX) with I/X/Z words take the
+    real-arithmetic forward/backward kernels: against the reverse oracle and
+    against the complex kernels of the same batch."""
+    r = np.random.default_rng(700 + seed)
+    n = int(r.integers(3, 10))
+    ops, P, bond = [], 0, [1] * (n + 1)
+    for _ in range(60):
+        kind = r.integers(0, 4)
+        q = int(r.integers(0, n))
+        if kind == 0:
+            ops.append(('ry', (q,), P))
+            P += 1
+        elif kind == 1:
+            ops.append((str(r.choice(['h', 'x', 'z'])), (q,), None))
+        elif q + 1 < n and bond[q + 1] < 16:
+            bond[q + 1] *= 2
+            ops.append(('cx', (q, q + 1) if r.integers(0, 2) else (q + 1, q), None))
+    words = [''.join(r.choice(list('IXZ'), n)) for _ in range(40)] + ['I' * n]
+    coeffs = r.normal(size=len(words))
+    X = r.uniform(-np.pi, np.pi, (3, P))
+    print(f'case: real tape seed={seed} n={n} P={P}', flush=True)
+    p = Problem(n, ops, P, words, coeffs, **SVD)
+    assert p.info()['real_arith'] == 1
+    b = Batch([p], [3])
+    e, g = b.value_grad(X)
+    e, g = e.cpu().numpy(), g.cpu().numpy().reshape(3, P)
+    b.set_arith('complex')
+    ec, gc = b.value_grad(X)
+    ec, gc = ec.cpu().numpy(), gc.cpu().numpy().reshape(3, P)
+    np.testing.assert_allclose(e, ec, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(g, gc, atol=1e-11, rtol=0)
+    for i in range(3):
+        eo, go = reverse_ref.value_gradient(n, ops, words, coeffs, X[i])
+        assert abs(e[i] - eo) < 1e-11 * max(1.0, abs(eo))
+        np.testing.assert_allclose(g[i], go, atol=1e-8, rtol=0)
+
+
+def test_real_path_40q_matches_complex(cuda):
+    """The 40-qubit workload's gradients, real kernels against complex kernels
+    (checkpoint stride 1 and 2)."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], **SVD)
+    X = w['x'][:2]
+    print('case: spsa-40q value_grad real vs complex kernels', flush=True)
+    b = Batch([p], [2])
+    e, g = b.value_grad(X)
+    b.set_checkpoint(2)
+    e2, g2 = b.value_grad(X)
+    assert np.array_equal(e.cpu().numpy(), e2.cpu().numpy()) and np.array_equal(g.cpu().numpy(), g2.cpu().numpy())
+    b.set_arith('complex')
+    ec, gc = b.value_grad(X)
+    np.testing.assert_allclose(e.cpu().numpy(), ec.cpu().numpy(), rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g.cpu().numpy(), gc.cpu().numpy(), atol=1e-9, rtol=0)
+
+
 def test_adjoint_60q_parameter_shift_and_fd(cuda):
     w = workloads.config(2)
     n, P = w['n'], w['nparams']
@@ -157,7 +214,10 @@ def test_batched_equals_plain_bitwise(cuda):
 
 
 def test_mixed_problem_batch(cuda):
-    """Instances of different problems in one batch match their own batches."""
+    """Instances of different problems in one batch match their own batches.
+    A batch runs real arithmetic only when all its problems qualify, so the
+    real 12-qubit problem runs complex here: it matches its own batch run
+    complex bitwise, and its real-arithmetic run to rounding."""
     a = workloads.config(0)
     r = np.random.default_rng(5)
     n = 9
@@ -170,9 +230,14 @@ def test_mixed_problem_batch(cuda):
     print('case: mixed 12q/9q value_grad batch', flush=True)
     e, g = Batch([pa, pb], [1, 1]).value_grad(np.concatenate([xa, xb]))
     e, g = e.cpu().numpy(), g.cpu().numpy()
-    ea, ga = Batch([pa], [1]).value_grad(xa)
+    ba = Batch([pa], [1])
+    ea_real, ga_real = ba.value_grad(xa)
+    ba.set_arith('complex')
+    ea, ga = ba.value_grad(xa)
     eb, gb = Batch([pb], [1]).value_grad(xb)
     assert e[0] == ea.cpu().numpy()[0] and e[1] == eb.cpu().numpy()[0]
+    assert abs(e[0] - float(ea_real.cpu()[0])) < 1e-12 * abs(e[0])
+    np.testing.assert_allclose(g[:a['nparams']], ga_real.cpu().numpy(), atol=1e-11, rtol=0)
     assert np.array_equal(g[:a['nparams']], ga.cpu().numpy())
     assert np.array_equal(g[a['nparams']:], gb.cpu().numpy())
     eo, go = reverse_ref.value_gradient(n, ops, words, coeffs, xb)
