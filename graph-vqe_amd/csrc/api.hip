@@ -197,7 +197,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->d_inst = upload(B->owned, B->inst.data(), B->inst.size());
   B->arena = upload<double2>(B->owned, nullptr, site);
   B->real = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->real_ok; });
-  if (B->real) B->rarena = upload<double>(B->owned, nullptr, site);
+  if (B->real) B->rarena = upload<double>(B->owned, nullptr, 2 * site);
   B->envs = upload<double2>(B->owned, nullptr, env);
   B->spills = upload<double2>(B->owned, nullptr, spill);
   B->terms = upload<double2>(B->owned, nullptr, term);
@@ -312,7 +312,8 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.leafs, B.spills,
+      hip_check(kl_rchains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total, B.leafs,
+                           B.spills,
                            terms ? B.terms : nullptr, B.parts),
                 "k_rchains launch");
     }
@@ -455,7 +456,8 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.spills, B.parts,
+      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+                                B.spills, B.parts,
                                 B.estore, B.gleafs),
                 "k_rchains_gfwd launch");
     }
@@ -465,7 +467,8 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     for (int g = B.max_gen - 1; g >= 0; --g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.spills, B.adjspill,
+      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+                               B.spills, B.adjspill,
                                B.estore, B.gleafs, B.aslot),
                 "k_rchains_bwd launch");
     }

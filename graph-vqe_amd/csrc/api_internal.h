@@ -46,7 +46,7 @@ struct Batch {
   // real-arithmetic energies: every problem has real gates and no Y words
   bool real = false;
   int arith = 0;              // 0: real path when `real`, 1: complex path always
-  double *rarena = nullptr;   // real parts of the site arena (real path)
+  double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   // captured evaluate + sample sequence
   hipStream_t cap_stream = nullptr;
@@ -104,16 +104,16 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 // kernel launchers (kernels_real.hip)
 hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena);
 hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                      const ProbDev *probs, const double *rarena, double2 *leafs, double2 *spills, double2 *terms,
-                      double *parts);
+                      const ProbDev *probs, const double *rarena, long long nsite, double2 *leafs, double2 *spills,
+                      double2 *terms, double *parts);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                           const ProbDev *probs, const double *rgarena, double2 *spills, double *parts, double2 *estore,
-                           double2 *gleafs);
+                           const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
+                           double2 *estore, double2 *gleafs);
 hipError_t kl_rchains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                          const ProbDev *probs, const double *rgarena, const double2 *spills, double2 *adjspill,
-                          const double2 *estore, const double2 *gleafs, double2 *aslot);
+                          const ProbDev *probs, const double *rgarena, long long nsite, const double2 *spills,
+                          double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot);
 hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                            const double2 *aslot, double2 *abar);
 // kernel launchers (kernels_grad.hip)

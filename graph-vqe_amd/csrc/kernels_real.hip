@@ -12,7 +12,10 @@
 //
 // 16 MFMAs per I/Z step instead of 48, 4 KiB site slots instead of 8 KiB, and
 // environments of 2 KiB (stored in the first half of the complex 4 KiB spill
-// and leaf slots).  The chain/forest schedule, work items, term ends and
+// and leaf slots).  The packed real arena holds every site twice: as A_i and
+// as A_i^T (suffix-forest chains read the transposed copy), so every fragment
+// load is a row load with 128-byte lines per 16 lanes.  The chain/forest
+// schedule, work items, term ends and
 // partial-energy layout are those of kernels_energy.hip, so k_reduce_energy /
 // k_reduce combine the results unchanged.
 #include "device_common.h"
@@ -54,16 +57,14 @@ __device__ __forceinline__ void renv_unit(int lane, REnv &e) {
   for (int t = 0; t < 4; ++t) e.v[t] = (lane == 0 && t == 0) ? 1.0 : 0.0;
 }
 
-// Real site slot layout: element (l, i, r) at (l*2 + i)*16 + r (512 doubles).
-__device__ __forceinline__ void rfrag_load(const double *slot, int lane, RFrag &f, bool transposed) {
+// Real site slot layout: element (l, i, r) at (l*2 + i)*16 + r (512 doubles);
+// in the transposed copy the slot holds A_i^T in the same layout.
+__device__ __forceinline__ void rfrag_load(const double *slot, int lane, RFrag &f) {
   const int j = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int row = kdx(g, t);
-      f.v[i][t] = slot[transposed ? ((j * 2 + i) * 16 + row) : ((row * 2 + i) * 16 + j)];
-    }
+    for (int t = 0; t < 4; ++t) f.v[i][t] = slot[(kdx(g, t) * 2 + i) * 16 + j];
 }
 
 // X_i = E M_i (D layout), the Gauss form's t1 of prod_x.
@@ -87,9 +88,9 @@ __device__ __forceinline__ d4 rprod_y(const d4 &x, const RFrag &f, int j) {
 template <bool TERMS>
 __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int nitems,
                                              const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-                                             const double *__restrict__ rarena, double2 *__restrict__ leafs,
-                                             double2 *__restrict__ spills, double2 *__restrict__ terms,
-                                             double *__restrict__ parts) {
+                                             const double *__restrict__ rarena, const double *__restrict__ rarenaT,
+                                             double2 *__restrict__ leafs, double2 *__restrict__ spills,
+                                             double2 *__restrict__ terms, double *__restrict__ parts) {
   const int lane = threadIdx.x & 63;
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
@@ -100,24 +101,23 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
-  const double *sites = rarena + inst.site_off;
+  const double *sites = (ch.dir != 0 ? rarenaT : rarena) + inst.site_off;
   double2 *spill = spills + inst.spill_off;
   double2 *tv = terms + inst.term_off;
 
   REnv e;
   if (ch.src >= 0) renv_load(rslot(spill, ch.src), lane, e);
   else renv_unit(lane, e);
-  const bool transposed = ch.dir != 0;
   double acc = 0.0;
 
   StepDev sd0 = uload(P.steps + ch.step0);
   RFrag fa, fb;
-  rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa, transposed);
+  rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa);
   // one chain step on fragment f; prefetches the next step's fragment into nf
   auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
     if (st + 1 < ch.nsteps) {
       nsd = uload(P.steps + ch.step0 + st + 1);
-      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, transposed);
+      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     }
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
     REnv cont;
@@ -213,11 +213,10 @@ __device__ __forceinline__ int sidx(int l, int i, int r) { return (l * 2 + i) * 
 // Input environment of the step after `step` in its chain (the continuing
 // child), with the products and combination of k_rchains_gfwd: bit for bit
 // the environment the forward pass had.
-__device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, int step, bool transposed, int lane,
-                                         REnv &e) {
+__device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, int step, int lane, REnv &e) {
   const StepDev sd = uload(P.steps + step);
   RFrag f;
-  rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+  rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, f);
   int kind = CK_I;
   for (int c = 0; c < sd.nchild; ++c) {
     const ChildDev cd = uload(P.children + sd.child0 + c);
@@ -245,8 +244,9 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
 // prefix leaves the backward pass reads (k_chains_gfwd in real arithmetic).
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena, double2 *__restrict__ spills,
-               double *__restrict__ parts, double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
+               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+               const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
+               double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
   const int lane = threadIdx.x & 63;
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
@@ -257,22 +257,21 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
-  const double *sites = rgarena + inst.site_off;
+  const double *sites = (ch.dir != 0 ? rgarenaT : rgarena) + inst.site_off;
   double2 *spill = spills + inst.spill_off;
   double2 *es = estore + inst.ck_off * GV_ENV;
   double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
   REnv e;
   if (ch.src >= 0) renv_load(rslot(spill, ch.src), lane, e);
   else renv_unit(lane, e);
-  const bool transposed = ch.dir != 0;
   double acc = 0.0;
   StepDev sd0 = uload(P.steps + ch.step0);
   RFrag fa, fb;
-  rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa, transposed);
+  rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa);
   auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
     if (st + 1 < ch.nsteps) {
       nsd = uload(P.steps + ch.step0 + st + 1);
-      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, transposed);
+      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     }
     const int cs = uload(P.ck_slot + ch.step0 + st);
     if (cs >= 0) renv_store(rslot(es, cs), lane, e);
@@ -323,7 +322,7 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
-              const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
+              const double *__restrict__ rgarenaT, const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
               const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
   __shared__ double tr[4][16][17];
   const int lane = threadIdx.x & 63;
@@ -337,7 +336,9 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
-  const double *sites = rgarena + inst.site_off;
+  // M_i as the forward pass read it (radvance) and in the A-operand form (its transpose)
+  const double *sites = (ch.dir != 0 ? rgarenaT : rgarena) + inst.site_off;
+  const double *sites_a = (ch.dir != 0 ? rgarena : rgarenaT) + inst.site_off;
   const double2 *spill = spills + inst.spill_off;
   double2 *aspill = adjspill + inst.spill_off;
   const double2 *es = estore + inst.ck_off * GV_ENV;
@@ -360,10 +361,10 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
       int s0 = st;
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
       renv_load(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
-      for (int q = s0; q < st; ++q) radvance(P, sites, ch.step0 + q, transposed, lane, E);
+      for (int q = s0; q < st; ++q) radvance(P, sites, ch.step0 + q, lane, E);
     }
     RFrag fa;  // M_i in the A-operand form
-    rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, fa, !transposed);
+    rfrag_load(sites_a + (size_t)sd.site * GV_SLOT, lane, fa);
     REnv y00, y11, y01;
 #pragma unroll
     for (int t = 0; t < 4; ++t) y00.v[t] = y11.v[t] = y01.v[t] = 0.0;
@@ -497,25 +498,30 @@ k_rsite_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pr
 }
 
 // Real parts of the prepared site arena (imaginary parts are exactly zero on
-// this path).
+// this path), as A_i into rarena and as A_i^T into rarenaT.
 extern "C" __global__ void __launch_bounds__(256)
-k_pack_real(const double2 *__restrict__ arena, double *__restrict__ rarena, long long n) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    rarena[i] = arena[i].x;
+k_pack_real(const double2 *__restrict__ arena, double *__restrict__ rarena, double *__restrict__ rarenaT,
+            long long n) {
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    rarena[e] = arena[e].x;
+    const long long base = e - (e % GV_SLOT);
+    const int k = (int)(e % GV_SLOT), row = k >> 5, i = (k >> 4) & 1, j = k & 15;  // A^T[row][i][j] = A[j][i][row]
+    rarenaT[e] = arena[base + (j * 2 + i) * 16 + row].x;
+  }
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-          const ProbDev *__restrict__ probs, const double *__restrict__ rarena, double2 *__restrict__ leafs,
-          double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  rchains_body<true>(items, nitems, insts, probs, rarena, leafs, spills, terms, nullptr);
+          const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+          double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  rchains_body<true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-                 const ProbDev *__restrict__ probs, const double *__restrict__ rarena, double2 *__restrict__ leafs,
-                 double2 *__restrict__ spills, double *__restrict__ parts) {
-  rchains_body<false>(items, nitems, insts, probs, rarena, leafs, spills, nullptr, parts);
+                 const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                 double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
+  rchains_body<false>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
@@ -536,35 +542,35 @@ namespace gv {
 
 hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena) {
   const long long blocks = std::min<long long>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_pack_real, dim3((unsigned)blocks), dim3(256), 0, s, arena, rarena, n);
+  hipLaunchKernelGGL(k_pack_real, dim3((unsigned)blocks), dim3(256), 0, s, arena, rarena, rarena + n, n);
   return hipGetLastError();
 }
 
 hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                      const ProbDev *probs, const double *rarena, double2 *leafs, double2 *spills, double2 *terms,
-                      double *parts) {
+                      const ProbDev *probs, const double *rarena, long long nsite, double2 *leafs, double2 *spills,
+                      double2 *terms, double *parts) {
   if (terms)
     hipLaunchKernelGGL(k_rchains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
-                       rarena, leafs, spills, terms);
+                       rarena, nsite, leafs, spills, terms);
   else
     hipLaunchKernelGGL(k_rchains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts,
-                       probs, rarena, leafs, spills, parts);
+                       probs, rarena, nsite, leafs, spills, parts);
   return hipGetLastError();
 }
 
 hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                           const ProbDev *probs, const double *rgarena, double2 *spills, double *parts, double2 *estore,
-                           double2 *gleafs) {
+                           const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
+                           double2 *estore, double2 *gleafs) {
   hipLaunchKernelGGL(k_rchains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
-                     rgarena, spills, parts, estore, gleafs);
+                     rgarena, rgarena + nsite, spills, parts, estore, gleafs);
   return hipGetLastError();
 }
 
 hipError_t kl_rchains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                          const ProbDev *probs, const double *rgarena, const double2 *spills, double2 *adjspill,
-                          const double2 *estore, const double2 *gleafs, double2 *aslot) {
+                          const ProbDev *probs, const double *rgarena, long long nsite, const double2 *spills,
+                          double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot) {
   hipLaunchKernelGGL(k_rchains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
-                     rgarena, spills, adjspill, estore, gleafs, aslot);
+                     rgarena, rgarena + nsite, spills, adjspill, estore, gleafs, aslot);
   return hipGetLastError();
 }
 
