@@ -195,7 +195,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->n_site_total = site;
   B->n_spill_total = spill;
   B->d_inst = upload(B->owned, B->inst.data(), B->inst.size());
-  B->arena = upload<double2>(B->owned, nullptr, site);
+  B->arena = upload<double2>(B->owned, nullptr, 2 * site);
   B->real = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->real_ok; });
   if (B->real) B->rarena = upload<double>(B->owned, nullptr, 2 * site);
   B->envs = upload<double2>(B->owned, nullptr, env);
@@ -326,14 +326,17 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     B.mark(4, stream);
     return;
   }
+  hip_check(kl_transpose_sites(B.n_site_total, stream, B.arena), "k_transpose_sites launch");
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
     if (terms)
-      hip_check(kl_chains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.leafs, B.spills, B.terms),
+      hip_check(kl_chains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total, B.leafs,
+                          B.spills, B.terms),
                 "k_chains launch");
     else
-      hip_check(kl_chains_energy(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.leafs, B.spills,
+      hip_check(kl_chains_energy(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total,
+                                 B.leafs, B.spills,
                                  B.parts),
                 "k_chains_energy launch");
   }
@@ -435,7 +438,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   if (!params && B.n_params_total > 0) throw std::runtime_error("null parameter array");
   const int ninst = (int)B.inst.size();
   if (!B.grad_ready) {
-    B.garena = upload<double2>(B.owned, nullptr, B.n_site_total);
+    B.garena = upload<double2>(B.owned, nullptr, 2 * B.n_site_total);
     B.abar = upload<double2>(B.owned, nullptr, B.n_site_total);
     B.rotin = upload<double2>(B.owned, nullptr, (size_t)B.n_rot_total * GV_SLOT);
     B.rgrad = upload<double>(B.owned, nullptr, B.n_rot_total);
@@ -482,11 +485,13 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.mark(6, stream);
     return;
   }
+  hip_check(kl_transpose_sites(B.n_site_total, stream, B.garena), "k_transpose_sites launch");
   B.mark(1, stream);
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.spills, B.parts, B.estore,
+    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+                             B.parts, B.estore,
                              B.gleafs),
               "k_chains_gfwd launch");
   }
@@ -496,7 +501,8 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   for (int g = B.max_gen - 1; g >= 0; --g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.spills, B.adjspill, B.estore,
+    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+                            B.adjspill, B.estore,
                             B.gleafs, B.aslot),
               "k_chains_bwd launch");
   }

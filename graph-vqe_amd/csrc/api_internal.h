@@ -20,6 +20,8 @@ struct Batch {
   std::vector<void *> owned;  // every device allocation of this batch
   ProbDev *d_probs = nullptr;
   InstDev *d_inst = nullptr;
+  // site arena: n_site_total elements of the prepared MPS, then (complex
+  // forests) the same sites transposed
   double2 *arena = nullptr, *envs = nullptr, *spills = nullptr, *terms = nullptr;
   double *parts = nullptr;  // chain partial energies
   int *status = nullptr, *bonds = nullptr, *stats = nullptr;
@@ -89,15 +91,16 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms);
 hipError_t kl_prep(int ninst, bool real, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                    const double *params, double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
+hipError_t kl_transpose_sites(long long n, hipStream_t s, double2 *arena);
 hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                     const double2 *arena, double2 *leafs, double2 *spills, double2 *terms);
+                     const double2 *arena, long long nsite, double2 *leafs, double2 *spills, double2 *terms);
 hipError_t kl_leaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                          const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
                      double *energies);
 hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                            const ProbDev *probs, const double2 *arena, double2 *leafs, double2 *spills,
-                            double *parts);
+                            const ProbDev *probs, const double2 *arena, long long nsite, double2 *leafs,
+                            double2 *spills, double *parts);
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                             const double *parts, double *energies, int with_leaves);
 void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
@@ -120,10 +123,11 @@ hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *i
 hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const double *params, double2 *garena, double2 *rotin, int *status);
 hipError_t kl_chains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                          const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs);
+                          const double2 *garena, long long nsite, double2 *spills, double *parts, double2 *estore,
+                          double2 *gleafs);
 hipError_t kl_chains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                         const double2 *garena, const double2 *spills, double2 *adjspill, const double2 *estore,
-                         const double2 *gleafs, double2 *aslot);
+                         const double2 *garena, long long nsite, const double2 *spills, double2 *adjspill,
+                         const double2 *estore, const double2 *gleafs, double2 *aslot);
 hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const double2 *aslot, double2 *abar);
 hipError_t kl_site_bwd(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,

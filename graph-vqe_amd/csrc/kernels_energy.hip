@@ -55,9 +55,9 @@ k_envs(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict__
 template <bool TERMS>
 __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int nitems,
                                             const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-                                            const double2 *__restrict__ arena, double2 *__restrict__ leafs,
-                                            double2 *__restrict__ spills, double2 *__restrict__ terms,
-                                            double *__restrict__ parts) {
+                                            const double2 *__restrict__ arena, long long nsite,
+                                            double2 *__restrict__ leafs, double2 *__restrict__ spills,
+                                            double2 *__restrict__ terms, double *__restrict__ parts) {
   const int lane = threadIdx.x & 63;
   // XCD-aware block order: hardware block b runs on XCD b % 8; give each XCD
   // a contiguous range of work items so one instance's chains share an L2.
@@ -71,7 +71,8 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
   const int n = inst.n;
-  const double2 *sites = arena + inst.site_off;
+  // suffix-forest chains read the transposed copy (second half of the arena)
+  const double2 *sites = arena + (ch.dir != 0 ? nsite : 0) + inst.site_off;
   double2 *spill = spills + inst.spill_off;
   double2 *tv = terms + inst.term_off;
 
@@ -81,19 +82,17 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   Env e;
   if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
   else env_unit(lane, e);
-  const bool transposed = ch.dir != 0;
-
   double acc = 0.0;  // energy mode: this lane's share of sum_k c_k <P_k>
   StepDev sd = uload(P.steps + ch.step0);
   Frag f;
-  frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+  frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, false);
   for (int st = 0; st < ch.nsteps; ++st) {
     // prefetch the next step's site tensor while this step computes
     StepDev nsd = sd;
     Frag nf;
     if (st + 1 < ch.nsteps) {
       nsd = uload(P.steps + ch.step0 + st + 1);
-      frag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, transposed);
+      frag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, false);
     }
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
@@ -167,16 +166,27 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
 
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-         const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, double2 *__restrict__ leafs,
-         double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  chains_body<true>(items, nitems, insts, probs, arena, leafs, spills, terms, nullptr);
+         const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, long long nsite,
+         double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  chains_body<true>(items, nitems, insts, probs, arena, nsite, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-                const ProbDev *__restrict__ probs, const double2 *__restrict__ arena,
+                const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, long long nsite,
                 double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
-  chains_body<false>(items, nitems, insts, probs, arena, leafs, spills, nullptr, parts);
+  chains_body<false>(items, nitems, insts, probs, arena, nsite, leafs, spills, nullptr, parts);
+}
+
+// Transposed copy of the site arena: slot s of the second half holds A_i^T
+// (element (l, i, r) = A[r][i][l]), so suffix-forest chains load row fragments.
+extern "C" __global__ void __launch_bounds__(256)
+k_transpose_sites(double2 *__restrict__ arena, long long n) {
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const long long base = e - (e % GV_SLOT);
+    const int k = (int)(e % GV_SLOT), row = k >> 5, i = (k >> 4) & 1, j = k & 15;
+    arena[n + e] = arena[base + (j * 2 + i) * 16 + row];
+  }
 }
 
 // Term ends of the deferred prefix leaves: one wave per (instance, leaf),
@@ -322,10 +332,16 @@ hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2
   return hipGetLastError();
 }
 
+hipError_t kl_transpose_sites(long long n, hipStream_t s, double2 *arena) {
+  const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_transpose_sites, dim3((unsigned)blocks), dim3(256), 0, s, arena, n);
+  return hipGetLastError();
+}
+
 hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                     const double2 *arena, double2 *leafs, double2 *spills, double2 *terms) {
+                     const double2 *arena, long long nsite, double2 *leafs, double2 *spills, double2 *terms) {
   hipLaunchKernelGGL(k_chains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
-                     leafs, spills, terms);
+                     nsite, leafs, spills, terms);
   return hipGetLastError();
 }
 
@@ -342,10 +358,10 @@ hipError_t kl_leaf_terms(int nitems, hipStream_t s, const int2 *items, const Ins
 }
 
 hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
-                            const ProbDev *probs, const double2 *arena, double2 *leafs, double2 *spills,
-                            double *parts) {
+                            const ProbDev *probs, const double2 *arena, long long nsite, double2 *leafs,
+                            double2 *spills, double *parts) {
   hipLaunchKernelGGL(k_chains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
-                     leafs, spills, parts);
+                     nsite, leafs, spills, parts);
   return hipGetLastError();
 }
 

@@ -134,8 +134,9 @@ k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
 // terms against a suffix-forest slot.
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-              const ProbDev *__restrict__ probs, const double2 *__restrict__ garena, double2 *__restrict__ spills,
-              double *__restrict__ parts, double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
+              const ProbDev *__restrict__ probs, const double2 *__restrict__ garena, long long nsite,
+              double2 *__restrict__ spills, double *__restrict__ parts, double2 *__restrict__ estore,
+              double2 *__restrict__ gleafs) {
   const int lane = threadIdx.x & 63;
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
@@ -146,20 +147,19 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
-  const double2 *sites = garena + inst.site_off;
+  const double2 *sites = garena + (ch.dir != 0 ? nsite : 0) + inst.site_off;  // transposed copy for suffix chains
   double2 *spill = spills + inst.spill_off;
   double2 *es = estore + inst.ck_off * GV_ENV;
   double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
   Env e;
   if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
   else env_unit(lane, e);
-  const bool transposed = ch.dir != 0;
   double acc = 0.0;
   for (int st = 0; st < ch.nsteps; ++st) {
     const int step = ch.step0 + st;
     const StepDev sd = uload(P.steps + step);
     Frag f;
-    frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+    frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, false);
     const int cs = uload(P.ck_slot + step);
     if (cs >= 0) env_store(es + (size_t)cs * GV_ENV, lane, e);
     d4 x0r, x0i, x1r, x1i;
@@ -244,11 +244,11 @@ __device__ __forceinline__ void cacc_get(const CAcc &c, d4 &dr, d4 &di) {
 // Input environment of the step after `step` in its chain: the forward
 // transfer of `step`'s continuing child (action -2), computed with the same
 // products in the same order as k_chains_gfwd (so bit for bit the same).
-__device__ __forceinline__ void advance(const ProbDev &P, const double2 *sites, int step, bool transposed,
+__device__ __forceinline__ void advance(const ProbDev &P, const double2 *sites, int step,
                                         int lane, Env &e) {
   const StepDev sd = uload(P.steps + step);
   Frag f;
-  frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, transposed);
+  frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, false);
   int kind = CK_I;
   for (int c = 0; c < sd.nchild; ++c) {
     const ChildDev cd = uload(P.children + sd.child0 + c);
@@ -307,7 +307,7 @@ __device__ __forceinline__ void add_conj(Env &e, const Env &q, double w) {
 //   Mbar_i = 2 E G_i            (E Hermitian)
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-             const ProbDev *__restrict__ probs, const double2 *__restrict__ garena,
+             const ProbDev *__restrict__ probs, const double2 *__restrict__ garena, long long nsite,
              const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
   __shared__ double2 tr[4][16][17];
@@ -322,7 +322,9 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
-  const double2 *sites = garena + inst.site_off;
+  // M_i as the forward pass read it (advance), and its transpose (A-operand form)
+  const double2 *sites = garena + (ch.dir != 0 ? nsite : 0) + inst.site_off;
+  const double2 *sites_a = garena + (ch.dir != 0 ? 0 : nsite) + inst.site_off;
   const double2 *spill = spills + inst.spill_off;
   double2 *aspill = adjspill + inst.spill_off;
   const double2 *es = estore + inst.ck_off * GV_ENV;
@@ -344,10 +346,10 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       int s0 = st;
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
       env_load(es + (size_t)uload(P.ck_slot + ch.step0 + s0) * GV_ENV, lane, E);
-      for (int q = s0; q < st; ++q) advance(P, sites, ch.step0 + q, transposed, lane, E);
+      for (int q = s0; q < st; ++q) advance(P, sites, ch.step0 + q, lane, E);
     }
     Frag fa;  // E layout of M_i (A operand form)
-    frag_load(sites + (size_t)sd.site * GV_SLOT, lane, fa, !transposed);
+    frag_load(sites_a + (size_t)sd.site * GV_SLOT, lane, fa, false);
     Env y00, y11, y01, y10;
     env_zero(y00);
     env_zero(y11);
@@ -604,17 +606,18 @@ hipError_t kl_build_local(int ninst, int max_n, hipStream_t s, const InstDev *in
 }
 
 hipError_t kl_chains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                          const double2 *garena, double2 *spills, double *parts, double2 *estore, double2 *gleafs) {
+                          const double2 *garena, long long nsite, double2 *spills, double *parts, double2 *estore,
+                          double2 *gleafs) {
   hipLaunchKernelGGL(k_chains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, garena,
-                     spills, parts, estore, gleafs);
+                     nsite, spills, parts, estore, gleafs);
   return hipGetLastError();
 }
 
 hipError_t kl_chains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                         const double2 *garena, const double2 *spills, double2 *adjspill, const double2 *estore,
-                         const double2 *gleafs, double2 *aslot) {
+                         const double2 *garena, long long nsite, const double2 *spills, double2 *adjspill,
+                         const double2 *estore, const double2 *gleafs, double2 *aslot) {
   hipLaunchKernelGGL(k_chains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, garena,
-                     spills, adjspill, estore, gleafs, aslot);
+                     nsite, spills, adjspill, estore, gleafs, aslot);
   return hipGetLastError();
 }
 
