@@ -184,6 +184,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       param += P.nparams;
       part += (long long)P.chains.size() + (long long)P.leaves.size();
       B->max_gen = std::max(B->max_gen, P.ngen);
+      B->max_ggen = std::max(B->max_ggen, P.nggen);
     }
   }
   if (B->inst.empty()) throw std::runtime_error("empty batch");
@@ -227,6 +228,17 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       return la > lb;
     });
     B->items_len.push_back(upload(B->owned, items.data(), items.size()));
+  }
+  for (int g = 0; g < B->max_ggen; ++g) {
+    std::vector<int2> items;
+    for (size_t i = 0; i < B->inst.size(); ++i) {
+      const Problem &P = *B->probs[B->inst[i].prob];
+      if (g >= P.nggen) continue;
+      for (int q = P.ggen_begin[g]; q < P.ggen_begin[g + 1]; ++q)
+        items.push_back(make_int2((int)i, P.ggen_chains[q]));
+    }
+    B->gitems.push_back(upload(B->owned, items.data(), items.size()));
+    B->gn_items.push_back((int)items.size());
   }
   B->set_checkpoint(1);
   return B.release();
@@ -456,10 +468,10 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     // real tape, no Y words: the forests and their adjoints in real arithmetic
     hip_check(kl_pack_real(B.n_site_total, stream, B.garena, B.rarena), "k_pack_real launch");
     B.mark(1, stream);
-    for (int g = 0; g < B.max_gen; ++g) {
-      const int ni = B.n_items[g];
+    for (int g = 0; g < B.max_ggen; ++g) {
+      const int ni = B.gn_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                 B.spills, B.parts,
                                 B.estore, B.gleafs),
                 "k_rchains_gfwd launch");
@@ -467,10 +479,10 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.mark(2, stream);
     hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 0), "k_reduce launch");
     B.mark(3, stream);
-    for (int g = B.max_gen - 1; g >= 0; --g) {
-      const int ni = B.n_items[g];
+    for (int g = B.max_ggen - 1; g >= 0; --g) {
+      const int ni = B.gn_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                B.spills, B.adjspill,
                                B.estore, B.gleafs, B.aslot),
                 "k_rchains_bwd launch");
@@ -487,10 +499,10 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   }
   hip_check(kl_transpose_sites(B.n_site_total, stream, B.garena), "k_transpose_sites launch");
   B.mark(1, stream);
-  for (int g = 0; g < B.max_gen; ++g) {
-    const int ni = B.n_items[g];
+  for (int g = 0; g < B.max_ggen; ++g) {
+    const int ni = B.gn_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
                              B.parts, B.estore,
                              B.gleafs),
               "k_chains_gfwd launch");
@@ -498,10 +510,10 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   B.mark(2, stream);
   hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 0), "k_reduce launch");
   B.mark(3, stream);
-  for (int g = B.max_gen - 1; g >= 0; --g) {
-    const int ni = B.n_items[g];
+  for (int g = B.max_ggen - 1; g >= 0; --g) {
+    const int ni = B.gn_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
                             B.adjspill, B.estore,
                             B.gleafs, B.aslot),
               "k_chains_bwd launch");

@@ -41,6 +41,10 @@ struct Batch {
   std::vector<int2 *> items_len;  // per generation, longest chains first across instances
   std::vector<int> n_items;
   int max_gen = 0;
+  // reverse-gradient generations (suffix forest, then prefix forest)
+  std::vector<int2 *> gitems;
+  std::vector<int> gn_items;
+  int max_ggen = 0;
   // launch configuration of the chain kernels (set by the autotuner)
   int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
   int order = 0;  // 0: instance-major items, 1: longest chains first

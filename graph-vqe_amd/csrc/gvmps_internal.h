@@ -97,6 +97,9 @@ struct ChildDev {
 // slot and k_leaf_terms contracts the terms [term0, term0 + nterm) of the
 // term-end list in parallel, off the chain's critical path.
 #define GV_DEFER_TERMS 3
+// Shallow prefix-forest depths with at most this many chains per instance run
+// in the suffix forest's energy launches (their term ends deferred).
+#define GV_MERGE_CHAINS 64
 struct LeafDev {
   int term0, nterm;
 };

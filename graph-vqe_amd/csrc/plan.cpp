@@ -471,7 +471,34 @@ void build_chains(Problem &P) {
   for (const ChainOut &c : chains) right_gens = std::max(right_gens, c.gen + 1);
   const size_t nright = chains.size();
   forest_chains(L, n, right_gens, P.nspill, P.flops_exec, chains);
-  // Order: generation ascending (suffix forest first), longest chains first.
+  // Reverse-gradient generations: the suffix forest, then the prefix forest
+  // (its inline term ends read suffix-forest slots).  Energy launches: the
+  // shallow prefix depths with few chains (latency-bound launches) run in the
+  // suffix forest's launches of the same depth, with every term end deferred
+  // to the leaf pass, which runs after both forests.
+  std::vector<int> depth_count;
+  for (size_t i = nright; i < chains.size(); ++i) {
+    const int d = chains[i].gen - right_gens;
+    if ((int)depth_count.size() <= d) depth_count.resize(d + 1, 0);
+    ++depth_count[d];
+  }
+  int merged = 0;
+  while (merged < (int)depth_count.size() && merged < right_gens && depth_count[merged] <= GV_MERGE_CHAINS) ++merged;
+  std::vector<int> ggen(chains.size()), early(chains.size(), 0);
+  for (size_t i = 0; i < chains.size(); ++i) {
+    ggen[i] = chains[i].gen;
+    if (i >= nright) {
+      const int d = chains[i].gen - right_gens;
+      if (d < merged) {
+        chains[i].gen = d;
+        early[i] = 1;
+      } else {
+        chains[i].gen = right_gens + d - merged;
+      }
+    }
+  }
+  // Order: energy launch ascending (suffix forest first within a launch),
+  // longest chains first.
   std::vector<int> order(chains.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -502,7 +529,7 @@ void build_chains(Problem &P) {
         cdv.term0 = (int)P.term_list.size();
         for (const TermEnd &te : F.nodes[node].ends) P.term_list.push_back(te);
         cdv.defer = -1;
-        if (cdv.nterm >= GV_DEFER_TERMS) {
+        if (cdv.nterm >= GV_DEFER_TERMS || (early[idx] && cdv.nterm > 0)) {
           cdv.defer = (int)P.leaves.size();
           P.leaves.push_back(LeafDev{cdv.term0, cdv.nterm});
         }
@@ -515,6 +542,20 @@ void build_chains(Problem &P) {
   }
   P.gen_begin.push_back((int)P.chains.size());
   P.ngen = (int)P.gen_begin.size() - 1;
+  // gradient generations over the same chain numbering, longest chains first
+  std::vector<int> pos(chains.size());
+  for (size_t k = 0; k < order.size(); ++k) pos[order[k]] = (int)k;
+  int nggen = 0;
+  for (int g : ggen) nggen = std::max(nggen, g + 1);
+  std::vector<std::vector<int>> byg(nggen);
+  for (int idx : order) byg[ggen[idx]].push_back(pos[idx]);
+  P.ggen_begin.assign(1, 0);
+  P.ggen_chains.clear();
+  for (const auto &v : byg) {
+    P.ggen_chains.insert(P.ggen_chains.end(), v.begin(), v.end());
+    P.ggen_begin.push_back((int)P.ggen_chains.size());
+  }
+  P.nggen = nggen;
 }
 
 // Per-site programs of the bond-doubling state used by the reverse gradient

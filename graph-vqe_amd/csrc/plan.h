@@ -40,7 +40,9 @@ struct Problem {
   std::vector<ChildDev> children;
   std::vector<TermEnd> term_list;
   int n_right_nodes = 0;                 // suffix-forest nodes
-  std::vector<int> gen_begin;            // chains of generation g: [gen_begin[g], gen_begin[g+1])
+  std::vector<int> gen_begin;            // energy launch g: chains [gen_begin[g], gen_begin[g+1])
+  std::vector<int> ggen_begin, ggen_chains;  // gradient generation g: chains ggen_chains[ggen_begin[g] ..)
+  int nggen = 0;
   int nspill = 0, ngen = 0, n_nodes = 0;
   std::vector<LeafDev> leaves;           // deferred term-end leaves (energy passes)
   // reverse gradient
