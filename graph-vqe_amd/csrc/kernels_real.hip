@@ -57,6 +57,27 @@ __device__ __forceinline__ void renv_unit(int lane, REnv &e) {
   for (int t = 0; t < 4; ++t) e.v[t] = (lane == 0 && t == 0) ? 1.0 : 0.0;
 }
 
+// Environments of I/X/Z words over a real MPS are symmetric.  The slots that
+// only term ends read (suffix-forest stores, deferred prefix leaves) keep the
+// upper triangle: element (a, k), a <= k, at a (31 - a) / 2 + k, 136 doubles
+// (9 of the 16 cache lines of a full 2 KiB environment).  Loads mirror it.
+__device__ __forceinline__ int sym_idx(int a, int k) { return a <= k ? a * (31 - a) / 2 + k : k * (31 - k) / 2 + a; }
+
+__device__ __forceinline__ void renv_store_sym(double *p, int lane, const REnv &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int k = kdx(g, t);
+    if (k >= a) p[sym_idx(a, k)] = e.v[t];
+  }
+}
+
+__device__ __forceinline__ void renv_load_sym(const double *p, int lane, REnv &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) e.v[t] = p[sym_idx(a, kdx(g, t))];
+}
+
 // Real site slot layout: element (l, i, r) at (l*2 + i)*16 + r (512 doubles);
 // in the transposed copy the slot holds A_i^T in the same layout.
 __device__ __forceinline__ void rfrag_load(const double *slot, int lane, RFrag &f) {
@@ -133,14 +154,14 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
         REnv ce;
 #pragma unroll
         for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
-        if (cd.store >= 0) renv_store(rslot(spill, cd.store), lane, ce);
-        if (cd.defer >= 0) renv_store(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
+        if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
+        if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
         const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
 #pragma unroll 2
         for (int t = 0; t < nterm_inline; ++t) {
           const TermEnd te = uload(P.term_list + cd.term0 + t);
           REnv r;
-          if (te.rslot >= 0) renv_load(rslot(spill, te.rslot), lane, r);
+          if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
           else renv_unit(lane, r);  // term ends at site n-1
           double sr = 0.0;
 #pragma unroll
@@ -184,13 +205,13 @@ __device__ __forceinline__ void rleaf_body(const int2 *__restrict__ items, int n
   const LeafDev lf = uload(P.leaves + it.y);
   const double2 *spill = spills + inst.spill_off;
   REnv e;
-  renv_load(rslot(leafs, inst.leaf_off + it.y), lane, e);
+  renv_load_sym(rslot(leafs, inst.leaf_off + it.y), lane, e);
   double acc = 0.0;
 #pragma unroll 4
   for (int t = 0; t < lf.nterm; ++t) {
     const TermEnd te = uload(P.term_list + lf.term0 + t);
     REnv r;
-    if (te.rslot >= 0) renv_load(rslot(spill, te.rslot), lane, r);
+    if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
     else renv_unit(lane, r);
     double sr = 0.0;
 #pragma unroll
