@@ -57,9 +57,9 @@ __device__ __forceinline__ void renv_unit(int lane, REnv &e) {
   for (int t = 0; t < 4; ++t) e.v[t] = (lane == 0 && t == 0) ? 1.0 : 0.0;
 }
 
-// Environments of I/X/Z words over a real MPS are symmetric.  The slots that
-// only term ends read (suffix-forest stores, deferred prefix leaves) keep the
-// upper triangle: element (a, k), a <= k, at a (31 - a) / 2 + k, 136 doubles
+// Environments of I/X/Z words over a real MPS are symmetric.  The energy
+// kernels' slots (chain hand-off spills, suffix-forest stores, deferred prefix
+// leaves) keep the upper triangle: element (a, k), a <= k, at a (31 - a) / 2 + k, 136 doubles
 // (9 of the 16 cache lines of a full 2 KiB environment).  Loads mirror it.
 __device__ __forceinline__ int sym_idx(int a, int k) { return a <= k ? a * (31 - a) / 2 + k : k * (31 - k) / 2 + a; }
 
@@ -127,7 +127,7 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
   double2 *tv = terms + inst.term_off;
 
   REnv e;
-  if (ch.src >= 0) renv_load(rslot(spill, ch.src), lane, e);
+  if (ch.src >= 0) renv_load_sym(rslot(spill, ch.src), lane, e);
   else renv_unit(lane, e);
   double acc = 0.0;
 
@@ -174,7 +174,7 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
           }
         }
         if (cd.action == -2) cont = ce;
-        else if (cd.action >= 0) renv_store(rslot(spill, cd.action), lane, ce);
+        else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
       }
     }
     e = cont;
