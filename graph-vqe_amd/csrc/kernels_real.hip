@@ -257,9 +257,12 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
 // The bond-doubling state of a real tape (RY, CX, ...) is real, and so are
 // the forest environments of I/X/Z words and their adjoints (the adjoint of a
 // real symmetric environment is real symmetric: conj and transpose drop out
-// of the complex formulas).  Same slots as kernels_grad.hip; environments use
-// the first 2 KiB of each 4 KiB slot, site adjoints the first 4 KiB of each
-// 8 KiB step slot.
+// of the complex formulas).  Same slots as kernels_grad.hip; site adjoints use
+// the first 4 KiB of each 8 KiB step slot.  Hand-off spills, suffix stores,
+// prefix leaves and adjoint spills are packed symmetric; the checkpointed
+// step environments stay full, so a recomputed step sees exactly the
+// environment the forward pass used and every checkpoint stride gives the
+// same bits.
 
 // Forward: energy chains keeping the checkpointed step environments and the
 // prefix leaves the backward pass reads (k_chains_gfwd in real arithmetic).
@@ -283,7 +286,7 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
   double2 *es = estore + inst.ck_off * GV_ENV;
   double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
   REnv e;
-  if (ch.src >= 0) renv_load(rslot(spill, ch.src), lane, e);
+  if (ch.src >= 0) renv_load_sym(rslot(spill, ch.src), lane, e);
   else renv_unit(lane, e);
   double acc = 0.0;
   StepDev sd0 = uload(P.steps + ch.step0);
@@ -309,12 +312,12 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
         REnv ce;
 #pragma unroll
         for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
-        if (cd.store >= 0) renv_store(rslot(spill, cd.store), lane, ce);
-        if (cd.gleaf >= 0) renv_store(rslot(gl, cd.gleaf), lane, ce);
+        if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
+        if (cd.gleaf >= 0) renv_store_sym(rslot(gl, cd.gleaf), lane, ce);
         for (int t = 0; t < cd.nterm; ++t) {
           const TermEnd te = uload(P.term_list + cd.term0 + t);
           REnv r;
-          if (te.rslot >= 0) renv_load(rslot(spill, te.rslot), lane, r);
+          if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
           else renv_unit(lane, r);
           double sr = 0.0;
 #pragma unroll
@@ -322,7 +325,7 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
           acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c_k) <P_k>
         }
         if (cd.action == -2) cont = ce;
-        else if (cd.action >= 0) renv_store(rslot(spill, cd.action), lane, ce);
+        else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
       }
     }
     e = cont;
@@ -393,7 +396,7 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
       const ChildDev cd = uload(P.children + sd.child0 + c);
       REnv cb;
       if (cd.action == -2) cb = eb;
-      else if (cd.action >= 0) renv_load(rslot(aspill, cd.action), lane, cb);
+      else if (cd.action >= 0) renv_load_sym(rslot(aspill, cd.action), lane, cb);
       else {
 #pragma unroll
         for (int t = 0; t < 4; ++t) cb.v[t] = 0.0;
@@ -404,7 +407,7 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
         const double ck = uload(P.coeffs + 2 * te.term);
         if (te.rslot >= 0) {
           REnv r;
-          renv_load(rslot(spill, te.rslot), lane, r);
+          renv_load_sym(rslot(spill, te.rslot), lane, r);
 #pragma unroll
           for (int v = 0; v < 4; ++v) cb.v[v] += ck * r.v[v];
         } else if (lane == 0) {
@@ -417,7 +420,7 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
         for (int q = uload(P.seed_begin + cd.store); q < q1; ++q) {
           const SeedTerm sdt = uload(P.seed_terms + q);
           REnv le;
-          renv_load(rslot(gl, sdt.gleaf), lane, le);
+          renv_load_sym(rslot(gl, sdt.gleaf), lane, le);
           const double ck = uload(P.coeffs + 2 * sdt.term);
 #pragma unroll
           for (int v = 0; v < 4; ++v) cb.v[v] += ck * le.v[v];
@@ -481,7 +484,7 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
 #pragma unroll
     for (int t = 0; t < 4; ++t) eb.v[t] = er[t];
   }
-  if (ch.src >= 0) renv_store(rslot(aspill, ch.src), lane, eb);
+  if (ch.src >= 0) renv_store_sym(rslot(aspill, ch.src), lane, eb);
 }
 
 // Per (instance, site): sum of its steps' real adjoints in step order, written
