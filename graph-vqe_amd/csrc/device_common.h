@@ -98,6 +98,34 @@ __device__ __forceinline__ void env_store(double2 *p, int lane, const Env &e) {
   for (int t = 0; t < 4; ++t) q[kdx(g, t)] = make_double2(e.re[t], e.im[t]);
 }
 
+// Forest environments of Hermitian Pauli words are Hermitian, and so are
+// their adjoints.  The reverse-gradient kernels' hand-off slots keep the upper
+// triangle: element (a, k), a <= k, at a (31 - a) / 2 + k (136 entries, 2176
+// of 4096 bytes); loads take E[a][k] = conj(E[k][a]) below the diagonal.  The
+// complex energy chains keep full slots: they are at their register cap, and
+// the index arithmetic of packed slots makes them spill.
+__device__ __forceinline__ int herm_idx(int a, int k) { return a <= k ? a * (31 - a) / 2 + k : k * (31 - k) / 2 + a; }
+
+__device__ __forceinline__ void env_store_herm(double2 *p, int lane, const Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int k = kdx(g, t);
+    if (k >= a) p[herm_idx(a, k)] = make_double2(e.re[t], e.im[t]);
+  }
+}
+
+__device__ __forceinline__ void env_load_herm(const double2 *p, int lane, Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int k = kdx(g, t);
+    const double2 v = p[herm_idx(a, k)];
+    e.re[t] = v.x;
+    e.im[t] = k < a ? -v.y : v.y;
+  }
+}
+
 __device__ __forceinline__ void env_unit(int lane, Env &e) {  // [1] in the (0,0) corner
 #pragma unroll
   for (int t = 0; t < 4; ++t) {

@@ -152,7 +152,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
   double2 *es = estore + inst.ck_off * GV_ENV;
   double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
   Env e;
-  if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
+  if (ch.src >= 0) env_load_herm(spill + (size_t)ch.src * GV_ENV, lane, e);
   else env_unit(lane, e);
   double acc = 0.0;
   for (int st = 0; st < ch.nsteps; ++st) {
@@ -185,12 +185,12 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
         else if (cd.kind == CK_X) { ce.re[v] = y01r[v] + y10r[v]; ce.im[v] = y01i[v] + y10i[v]; }
         else { ce.re[v] = y01i[v] - y10i[v]; ce.im[v] = -y01r[v] + y10r[v]; }
       }
-      if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
-      if (cd.gleaf >= 0) env_store(gl + (size_t)cd.gleaf * GV_ENV, lane, ce);
+      if (cd.store >= 0) env_store_herm(spill + (size_t)cd.store * GV_ENV, lane, ce);
+      if (cd.gleaf >= 0) env_store_herm(gl + (size_t)cd.gleaf * GV_ENV, lane, ce);
       for (int t = 0; t < cd.nterm; ++t) {
         const TermEnd te = uload(P.term_list + cd.term0 + t);
         Env r;
-        if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+        if (te.rslot >= 0) env_load_herm(spill + (size_t)te.rslot * GV_ENV, lane, r);
         else env_unit(lane, r);
         double sr = 0.0;
 #pragma unroll
@@ -198,7 +198,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
         acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c_k) Re<P_k>
       }
       if (cd.action == -2) cont = ce;
-      else if (cd.action >= 0) env_store(spill + (size_t)cd.action * GV_ENV, lane, ce);
+      else if (cd.action >= 0) env_store_herm(spill + (size_t)cd.action * GV_ENV, lane, ce);
     }
     e = cont;
   }
@@ -359,7 +359,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       const ChildDev cd = uload(P.children + sd.child0 + c);
       Env cb;
       if (cd.action == -2) cb = eb;
-      else if (cd.action >= 0) env_load(aspill + (size_t)cd.action * GV_ENV, lane, cb);
+      else if (cd.action >= 0) env_load_herm(aspill + (size_t)cd.action * GV_ENV, lane, cb);
       else env_zero(cb);
       // term ends (prefix forest): Cbar += Re(c_k) conj(Rt_k)
       for (int t = 0; t < cd.nterm; ++t) {
@@ -367,7 +367,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
         const double ck = uload(P.coeffs + 2 * te.term);
         if (te.rslot >= 0) {
           Env r;
-          env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+          env_load_herm(spill + (size_t)te.rslot * GV_ENV, lane, r);
           add_conj(cb, r, ck);
         } else if (lane == 0) {
           cb.re[0] += ck;  // unit right environment
@@ -379,7 +379,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
         for (int q = uload(P.seed_begin + cd.store); q < q1; ++q) {
           const SeedTerm sdt = uload(P.seed_terms + q);
           Env le;
-          env_load(gl + (size_t)sdt.gleaf * GV_ENV, lane, le);
+          env_load_herm(gl + (size_t)sdt.gleaf * GV_ENV, lane, le);
           add_conj(cb, le, uload(P.coeffs + 2 * sdt.term));
         }
       }
@@ -468,7 +468,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       eb.im[t] = -ei[t];
     }
   }
-  if (ch.src >= 0) env_store(aspill + (size_t)ch.src * GV_ENV, lane, eb);
+  if (ch.src >= 0) env_store_herm(aspill + (size_t)ch.src * GV_ENV, lane, eb);
 }
 
 // ------------------------------------------------------- site adjoints

@@ -262,12 +262,14 @@ def test_checkpoint_strides_are_bitwise_identical(cuda):
     w = workloads.config(1)
     p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], **SVD)
     X = w['x'][:3]
-    b = Batch([p], [3])
-    print('case: spsa-40q value_grad checkpoint strides 1/2/3', flush=True)
-    e1, g1 = b.value_grad(X)
-    e1, g1 = e1.cpu().numpy(), g1.cpu().numpy()
-    for stride in (2, 3):
-        b.set_checkpoint(stride)
-        e, g = b.value_grad(X)
-        assert np.array_equal(e.cpu().numpy(), e1), stride
-        assert np.array_equal(g.cpu().numpy(), g1), stride
+    for arith in ('auto', 'complex'):  # real-arithmetic and complex kernels
+        b = Batch([p], [3])
+        b.set_arith(arith)
+        print(f'case: spsa-40q value_grad checkpoint strides 1/2/3 ({arith})', flush=True)
+        e1, g1 = b.value_grad(X)
+        e1, g1 = e1.cpu().numpy(), g1.cpu().numpy()
+        for stride in (2, 3):
+            b.set_checkpoint(stride)
+            e, g = b.value_grad(X)
+            assert np.array_equal(e.cpu().numpy(), e1), (arith, stride)
+            assert np.array_equal(g.cpu().numpy(), g1), (arith, stride)
