@@ -138,6 +138,25 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
   return __longlong_as_double((long long)readlane_u64((unsigned long long)__double_as_longlong(v), l));
 }
 
+template <int CTRL>
+__device__ __forceinline__ double dpp_add_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return v + __longlong_as_double((long long)(((unsigned long long)ohi << 32) | olo));
+}
+
+// Wave-uniform sum (fixed order): DPP within each row of 16 lanes, then the
+// four row sums by readlane.
+__device__ __forceinline__ double wave_sum_uniform(double v) {
+  v = dpp_add_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add_f64<0x141>(v);  // row_half_mirror
+  v = dpp_add_f64<0x140>(v);  // row_mirror
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
 // Wave-uniform maximum of a 64-bit key: DPP within each row of 16 lanes
 // (quad swaps, half-row and row mirrors), then the four row maxima by readlane.
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
@@ -184,9 +203,31 @@ __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, 
   const double2 xj = make_double2(readlane_f64(xl.x, bi), REAL ? 0.0 : readlane_f64(xl.y, bi));
   // v = x + e |x| e_j, e = x_j / |x_j|;  tau = 2 / (v^H v) = 1 / (|x| (|x| + |x_j|))
   const double alpha = sqrt(a2);
-  const double axj = sqrt(xj.x * xj.x + xj.y * xj.y);
-  const double2 e = axj > 0.0 ? make_double2(xj.x / axj, xj.y / axj) : make_double2(1.0, 0.0);
+  // REAL: |x_j| = fabs(x_j) and e = sign(x_j) (e = 1 at zero), no divisions
+  const double axj = REAL ? fabs(xj.x) : sqrt(xj.x * xj.x + xj.y * xj.y);
+  const double2 e = REAL ? make_double2(xj.x < 0.0 ? -1.0 : 1.0, 0.0)
+                         : (axj > 0.0 ? make_double2(xj.x / axj, xj.y / axj) : make_double2(1.0, 0.0));
   const double tau = 1.0 / (alpha * (alpha + axj));
+  if constexpr (REAL) {
+    // the reflector is the pivot column: broadcast it by readlane (SGPRs),
+    // then dot and update take it as a scalar operand; no LDS round trip
+    double v[32 - R0];
+#pragma unroll
+    for (int r = R0; r < 32; ++r) {
+      v[r - R0] = readlane_f64(col[r].x, bi);
+      if (r < R0 + 4) {
+        if (r < j) v[r - R0] = 0.0;
+        if (r == j) v[r - R0] = xj.x + e.x * alpha;
+      }
+    }
+    double dr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = R0; r < 32; ++r) dr[r & 3] = fma(v[r - R0], col[r].x, dr[r & 3]);
+    const double dx = tau * ((dr[0] + dr[1]) + (dr[2] + dr[3]));
+#pragma unroll
+    for (int r = R0; r < 32; ++r) col[r].x -= v[r - R0] * dx;
+    return true;
+  }
   if (lane == bi) {
 #pragma unroll
     for (int r = R0; r < 32; ++r) {
@@ -263,7 +304,7 @@ __device__ double wave_pivoted_qr(double2 (*W)[PW], double2 *vb, int m, int nc, 
   for (int r = 0; r < 32; ++r)
     if (own && r < m) W[r][lane] = col[r];
   wave_sync();
-  return wave_sum(tr);
+  return wave_sum_uniform(tr);
 }
 
 // One-sided Jacobi on `rows` rows of panel W (Theta block nc wide, the
@@ -396,7 +437,7 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
     const int row = idx / m, col = idx % m;
     W[row][nc + col] = make_double2(row == col ? 1.0 : 0.0, 0.0);
   }
-  frob = wave_sum(frob);
+  frob = wave_sum_uniform(frob);
   wave_sync();
   const unsigned long long t1 = clk();
 
@@ -491,8 +532,9 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
   for (int idx = lane; idx < GV_SLOT; idx += 64) {
     const int row = idx >> 5, I = (idx >> 4) & 1, c = idx & 15;
     double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
-    if (row < Dl && c < k) va = cconj(W[L.perm[w][c]][nc + row * 2 + I]);
-    if (row < k && c < Dr) vb = W[L.perm[w][row]][I * Dr + c];
+    // direct factorisation: the row order is the QR's (identity permutation)
+    if (row < Dl && c < k) va = cconj(W[direct ? c : L.perm[w][c]][nc + row * 2 + I]);
+    if (row < k && c < Dr) vb = W[direct ? row : L.perm[w][row]][I * Dr + c];
     wa[idx] = va;
     wb[idx] = vb;
   }
