@@ -87,13 +87,13 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   Frag f;
   frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, false);
   for (int st = 0; st < ch.nsteps; ++st) {
-    // prefetch the next step's site tensor while this step computes
-    StepDev nsd = sd;
+    // prefetch the next step's site tensor while this step computes.  The
+    // load is unconditional (the last step reloads its own site): with the
+    // same number of loads in flight on every path, the wait for this step's
+    // operands leaves the prefetch outstanding (vmcnt counts in order).
+    const StepDev nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));
     Frag nf;
-    if (st + 1 < ch.nsteps) {
-      nsd = uload(P.steps + ch.step0 + st + 1);
-      frag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, false);
-    }
+    frag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf, false);
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
     prod_x(e, f, 1, x1r, x1i);

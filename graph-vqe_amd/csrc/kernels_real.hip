@@ -134,12 +134,13 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
   StepDev sd0 = uload(P.steps + ch.step0);
   RFrag fa, fb;
   rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa);
-  // one chain step on fragment f; prefetches the next step's fragment into nf
+  // one chain step on fragment f; prefetches the next step's fragment into nf.
+  // The prefetch is unconditional (the last step reloads its own site): with
+  // the same number of loads in flight on every path, the wait for f leaves
+  // the prefetch outstanding instead of draining it (vmcnt counts in order).
   auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
-    if (st + 1 < ch.nsteps) {
-      nsd = uload(P.steps + ch.step0 + st + 1);
-      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
-    }
+    nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));
+    rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
     REnv cont;
     // pass 0: I/Z children from Y00, Y11; pass 1: X children from Y01, Y10
@@ -293,12 +294,12 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
   RFrag fa, fb;
   rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa);
   auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
-    if (st + 1 < ch.nsteps) {
-      nsd = uload(P.steps + ch.step0 + st + 1);
-      rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
-    }
+    // checkpoint store first: the prefetch below is then the youngest memory
+    // operation when this step waits for f
     const int cs = uload(P.ck_slot + ch.step0 + st);
     if (cs >= 0) renv_store(rslot(es, cs), lane, e);
+    nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));  // unconditional prefetch, as above
+    rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
     REnv cont;
 #pragma nounroll
