@@ -199,6 +199,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->arena = upload<double2>(B->owned, nullptr, 2 * site);
   B->real = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->real_ok; });
   if (B->real) B->rarena = upload<double>(B->owned, nullptr, 2 * site);
+  B->iz = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->iz_only; });
   B->envs = upload<double2>(B->owned, nullptr, env);
   B->spills = upload<double2>(B->owned, nullptr, spill);
   B->terms = upload<double2>(B->owned, nullptr, term);
@@ -324,7 +325,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total, B.leafs,
+      hip_check(kl_rchains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total, B.leafs,
                            B.spills,
                            terms ? B.terms : nullptr, B.parts),
                 "k_rchains launch");

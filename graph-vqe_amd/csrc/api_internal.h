@@ -51,6 +51,7 @@ struct Batch {
   int prep_mode = 0;  // 0: gates + SVDs (k_prep), 1: exact bond doubling (k_build_local)
   // real-arithmetic energies: every problem has real gates and no Y words
   bool real = false;
+  bool iz = false;    // every problem's forest has I/Z children only: k_rchains*_iz
   int arith = 0;              // 0: real path when `real`, 1: complex path always
   double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
@@ -110,7 +111,7 @@ hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, cons
 void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 // kernel launchers (kernels_real.hip)
 hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena);
-hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                       const ProbDev *probs, const double *rarena, long long nsite, double2 *leafs, double2 *spills,
                       double2 *terms, double *parts);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,

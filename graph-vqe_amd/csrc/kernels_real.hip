@@ -105,8 +105,12 @@ __device__ __forceinline__ d4 rprod_y(const d4 &x, const RFrag &f, int j) {
 }
 
 // One generation of prefix/suffix chains in real arithmetic; the same work
-// items and slot numbering as chains_body in kernels_energy.hip.
-template <bool TERMS>
+// items and slot numbering as chains_body in kernels_energy.hip.  IZ: every
+// step of the forest has I/Z children only (Hamiltonians without X words), so
+// the site fragment and the X_i products die once Y00 and Y11 are formed;
+// with fewer live registers the kernel runs more waves per SIMD.  The
+// arithmetic per instance is the same in both forms.
+template <bool TERMS, bool IZ>
 __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int nitems,
                                              const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
                                              const double *__restrict__ rarena, const double *__restrict__ rarenaT,
@@ -145,20 +149,20 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
     REnv cont;
     // pass 0: I/Z children from Y00, Y11; pass 1: X children from Y01, Y10
 #pragma nounroll
-    for (int pass = 0; pass < 2; ++pass) {
-      if (!(sd.need & (1 << pass))) continue;
+    for (int pass = 0; pass < (IZ ? 1 : 2); ++pass) {
+      if (!IZ && !(sd.need & (1 << pass))) continue;
       const d4 ya = rprod_y(pass ? x1 : x0, f, 0);
       const d4 yb = rprod_y(pass ? x0 : x1, f, 1);
       for (int c = 0; c < sd.nchild; ++c) {
         const ChildDev cd = uload(P.children + sd.child0 + c);
-        if ((cd.kind == CK_X) != (pass == 1)) continue;
+        if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
         REnv ce;
 #pragma unroll
         for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
         if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
         if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
         const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
-#pragma unroll 2
+#pragma unroll(IZ ? GV_IZ_TERM_UNROLL : 2)
         for (int t = 0; t < nterm_inline; ++t) {
           const TermEnd te = uload(P.term_list + cd.term0 + t);
           REnv r;
@@ -539,14 +543,29 @@ extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
           const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
           double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  rchains_body<true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, terms, nullptr);
+  rchains_body<true, false>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                  const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
                  double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
-  rchains_body<false>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
+  rchains_body<false, false>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
+}
+
+// Forests whose steps have I/Z children only.
+extern "C" __global__ void __launch_bounds__(256) GV_IZ_WAVES
+k_rchains_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+             const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+             double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  rchains_body<true, true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, terms, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_IZ_WAVES
+k_rchains_energy_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                    const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                    double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
+  rchains_body<false, true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
@@ -571,15 +590,16 @@ hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double
   return hipGetLastError();
 }
 
-hipError_t kl_rchains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                       const ProbDev *probs, const double *rarena, long long nsite, double2 *leafs, double2 *spills,
                       double2 *terms, double *parts) {
+  const dim3 grid((nitems + wpb - 1) / wpb), block(64 * wpb);
   if (terms)
-    hipLaunchKernelGGL(k_rchains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
-                       rarena, nsite, leafs, spills, terms);
+    hipLaunchKernelGGL(iz ? k_rchains_iz : k_rchains, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite,
+                       leafs, spills, terms);
   else
-    hipLaunchKernelGGL(k_rchains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts,
-                       probs, rarena, nsite, leafs, spills, parts);
+    hipLaunchKernelGGL(iz ? k_rchains_energy_iz : k_rchains_energy, grid, block, 0, s, items, nitems, insts, probs,
+                       rarena, nsite, leafs, spills, parts);
   return hipGetLastError();
 }
 

@@ -773,6 +773,7 @@ std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int 
                std::none_of(P->site_words.begin(), P->site_words.end(),
                             [](const std::string &w) { return w.find('Y') != std::string::npos; });
   build_chains(*P);
+  P->iz_only = std::none_of(P->steps.begin(), P->steps.end(), [](const StepDev &sd) { return (sd.need & 2) != 0; });
   build_grad_meta(*P);
   return P;
 }

@@ -34,6 +34,7 @@ struct Problem {
   std::vector<std::string> site_words;   // site order
   std::vector<int> active;               // non-identity terms
   bool real_ok = false;                  // real gate matrices and no Y in any word: real-arithmetic energies
+  bool iz_only = false;                  // every forest step has I/Z children only (no X or Y words)
   // prefix forest, chain schedule
   std::vector<ChainDev> chains;
   std::vector<StepDev> steps;
