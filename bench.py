@@ -95,6 +95,8 @@ def main():
     ap.add_argument('--no-uneven', action='store_true')
     ap.add_argument('--uneven-reps', type=int, default=5)
     ap.add_argument('--grad-steps', type=int, default=3)
+    ap.add_argument('--grad-checkpoint', type=int, default=3,
+                    help='40-qubit adjoint: keep every k-th forward step environment (the rest are recomputed)')
     ap.add_argument('--plain-grad-sample', type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -323,6 +325,9 @@ def bench_grads(args, prob, info, x, B):
     import torch
     from gvmps import Batch
     batch = Batch([prob], [B])
+    # every k-th forward step keeps its input environment; the backward pass
+    # recomputes the rest (bitwise the same gradients at every stride)
+    batch.set_checkpoint(args.grad_checkpoint)
     for _ in range(args.warmup):
         batch.value_grad(x)
     torch.cuda.synchronize()
@@ -336,8 +341,10 @@ def bench_grads(args, prob, info, x, B):
     rate = B * args.grad_steps / dt
     del batch
     torch.cuda.empty_cache()
+    single = Batch([prob], [1])
+    single.set_checkpoint(args.grad_checkpoint)
     plain_rate, same = plain_rate_of(
-        lambda b, v: b.value_grad(v), Batch([prob], [1]), x[: args.plain_grad_sample],
+        lambda b, v: b.value_grad(v), single, x[: args.plain_grad_sample],
         lambda i, r: torch.equal(r[0], e[i:i + 1]) and torch.equal(r[1], g.view(B, -1)[i]))
     c = max(calls, 1)
     names = ["bond_doubling_build", "forward_chains", "energy_reduce", "backward_chains", "site_reduce",
@@ -356,6 +363,7 @@ def bench_grads(args, prob, info, x, B):
         "plain_matches_batched_bitwise": same,
         "phase_ms_per_step": {n: round(phase_ms[i] / c, 3) for i, n in enumerate(names)},
         "arithmetic": "real" if real else "complex",
+        "checkpoint_stride": args.grad_checkpoint,
         "chains_tflops": round(achieved, 3),
         "chains_frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 4),
     }

@@ -32,6 +32,8 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 // the Y products, and contract one inline term end at a time: they fit six
 // waves per SIMD (80 registers; measured 2% faster than five, 4% than four).
 #define GV_IZ_WAVES __attribute__((amdgpu_waves_per_eu(6, 6)))
+// The per-term form keeps more live state: five waves.
+#define GV_FIVE_WAVES __attribute__((amdgpu_waves_per_eu(5, 5)))
 #define GV_IZ_TERM_UNROLL 1
 
 // Index of this wavefront in its workgroup as a wave-uniform value.  Without

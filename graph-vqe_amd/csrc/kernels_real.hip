@@ -554,7 +554,7 @@ k_rchains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__re
 }
 
 // Forests whose steps have I/Z children only.
-extern "C" __global__ void __launch_bounds__(256) GV_IZ_WAVES
+extern "C" __global__ void __launch_bounds__(256) GV_FIVE_WAVES
 k_rchains_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
              const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
              double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
