@@ -208,7 +208,8 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                      "traffic": measured_traffic(),
                      "traffic_basis": "HBM bytes per step (all generations), rocprofv3 PMC, copy-calibrated",
-                     "kernel": ("k_rchains_energy (real arithmetic)" if real else "k_chains_energy")
+                     "kernel": (("k_rchains_energy_iz (real arithmetic, I/Z-only forest)" if info['iz_forest']
+                                 else "k_rchains_energy (real arithmetic)") if real else "k_chains_energy")
                                + " (all generations of one step) + leaf term-end kernel",
                      "algorithmic_flops_per_step": node_flops * B,
                      "executed_tflops": round(executed, 3)},

@@ -622,6 +622,7 @@ int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
     info->n_rotations = (int)P.rot_param.size();
     info->grad_ok = P.grad_error.empty() ? 1 : 0;
     info->real_arith = P.real_ok ? 1 : 0;
+    info->iz_forest = P.iz_only ? 1 : 0;
   })
 }
 

@@ -55,6 +55,7 @@ typedef struct gvmps_problem_info {
   int32_t n_rotations;             /* parameterised + constant rotations of the tape */
   int32_t grad_ok;                 /* 1: the tape has a bond-doubling reverse gradient */
   int32_t real_arith;              /* 1: real gate matrices and no Y words (real-arithmetic energies) */
+  int32_t iz_forest;               /* 1: no X or Y words: every forest step has I/Z children only */
 } gvmps_problem_info;
 
 /* Last error message of the calling thread ("" if none). */

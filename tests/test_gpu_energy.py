@@ -71,10 +71,12 @@ def test_random_circuits_xyz_words_vs_dense(cuda, seed):
         assert rel_close(e[i], d.energy(words, coeffs))
 
 
+@pytest.mark.parametrize('alphabet', ['IXZ', 'IZ'])
 @pytest.mark.parametrize('seed', range(3))
-def test_real_circuits_real_path_vs_complex_and_dense(cuda, seed):
-    """Real gates on |0..0> and I/X/Z words run the real-arithmetic forests;
-    they must agree with the complex kernels and the dense statevector."""
+def test_real_circuits_real_path_vs_complex_and_dense(cuda, seed, alphabet):
+    """Real gates on |0..0> and I/X/Z words run the real-arithmetic forests
+    (I/Z-only words: the I/Z chain kernels); they must agree with the complex
+    kernels and the dense statevector."""
     r = np.random.default_rng(300 + seed)
     n = int(r.integers(4, 10))
     ops, P = [], 0
@@ -89,10 +91,10 @@ def test_real_circuits_real_path_vs_complex_and_dense(cuda, seed):
         else:
             t = int(r.integers(0, n - 1))
             ops.append((str(r.choice(['cx', 'cx', 'swap'])), (q, t + 1 if t >= q else t), None))
-    words = [''.join(r.choice(list('IXZ'), n)) for _ in range(50)] + ['I' * n]
+    words = [''.join(r.choice(list(alphabet), n)) for _ in range(50)] + ['I' * n]
     coeffs = r.normal(size=len(words))
     X = r.uniform(-np.pi, np.pi, (4, P))
-    print(f'case: real circuit seed={seed} n={n} P={P}', flush=True)
+    print(f'case: real circuit seed={seed} n={n} P={P} words {alphabet}', flush=True)
     p = Problem(n, ops, P, words, coeffs, max_bond=16, abs_cut=0, rel_cut=0)
     assert p.info()['real_arith'] == 1
     b = Batch([p], [4])

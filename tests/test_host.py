@@ -66,6 +66,10 @@ def test_real_arithmetic_rules():
     for extra in [('rx', (0,), 0), ('rz', (2,), 0), ('s', (1,), None), ('y', (0,), None), ('rx', (3,), 0.3)]:
         assert Problem(4, real_ops + [extra], 1, words, coeffs).info()['real_arith'] == 0, extra
     assert Problem(4, real_ops, 1, ['ZYII'], [1.0]).info()['real_arith'] == 0
+    # iz_forest: no X or Y letter in any word (the I/Z chain kernels)
+    assert Problem(4, real_ops, 1, words, coeffs).info()['iz_forest'] == 0
+    assert Problem(4, real_ops, 1, ['ZIIZ', 'IZZI', 'IIII'], [1.0, 2.0, 0.5]).info()['iz_forest'] == 1
+    assert Problem(4, real_ops, 1, ['ZIIZ', 'IIXI'], [1.0, 2.0]).info()['iz_forest'] == 0
 
 
 def test_planner_rejects_bad_input():
