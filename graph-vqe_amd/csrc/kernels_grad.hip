@@ -447,20 +447,21 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // d is 2 G_i as the B operand and 2 G_i^T as the A operand, and E in
+      // its layout is E^T as the B operand: Mbar_i = E (2 G_i), and for suffix
+      // chains (whose slot holds the adjoint of A_i^T) Mbar_i^T = (2 G_i^T) E^T.
+      // D layout either way, so lanes with consecutive columns store
+      // consecutive slot elements.
       d4 mr, mi;
       {
         CAcc ma;
         cacc_zero(ma);
-        cmm(E.re, E.im, dr, di, false, false, ma);
+        if (transposed) cmm(dr, di, E.re, E.im, false, false, ma);
+        else cmm(E.re, E.im, dr, di, false, false, ma);
         cacc_get(ma, mr, mi);
       }
-      // D layout: lane (r + 16 g), register v = Mbar_i[kdx(g, v)][r]
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = kdx(g4, v);
-        const int idx = transposed ? sidx(a16, i, row) : sidx(row, i, a16);
-        slot[idx] = make_double2(mr[v], mi[v]);
-      }
+      for (int v = 0; v < 4; ++v) slot[sidx(kdx(g4, v), i, a16)] = make_double2(mr[v], mi[v]);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {

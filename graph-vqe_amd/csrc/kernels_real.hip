@@ -477,14 +477,21 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // d is 2 G_i as the B operand and 2 G_i^T as the A operand, and E in
+      // its layout is E^T as the B operand: Mbar_i = E (2 G_i), and for suffix
+      // chains (whose slot holds the adjoint of A_i^T) Mbar_i^T = (2 G_i^T) E^T.
+      // D layout either way, so lanes with consecutive columns store
+      // consecutive slot elements.
       d4 m = {0.0, 0.0, 0.0, 0.0};
+      if (transposed) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) m = mfma(E.v[t], d[t], m);
+        for (int t = 0; t < 4; ++t) m = mfma(d[t], E.v[t], m);
+      } else {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = kdx(g4, v);
-        slot[transposed ? sidx(a16, i, row) : sidx(row, i, a16)] = m[v];
+        for (int t = 0; t < 4; ++t) m = mfma(E.v[t], d[t], m);
       }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) slot[sidx(kdx(g4, v), i, a16)] = m[v];
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) eb.v[t] = er[t];
