@@ -95,7 +95,7 @@ def main():
     ap.add_argument('--no-uneven', action='store_true')
     ap.add_argument('--uneven-reps', type=int, default=5)
     ap.add_argument('--grad-steps', type=int, default=3)
-    ap.add_argument('--grad-checkpoint', type=int, default=2,
+    ap.add_argument('--grad-checkpoint', type=int, default=1,
                     help='40-qubit adjoint: keep every k-th forward step environment (the rest are recomputed)')
     ap.add_argument('--plain-grad-sample', type=int, default=16)
     args = ap.parse_args()

@@ -105,6 +105,25 @@ __device__ __forceinline__ void env_store(double2 *p, int lane, const Env &e) {
   for (int t = 0; t < 4; ++t) q[kdx(g, t)] = make_double2(e.re[t], e.im[t]);
 }
 
+// Checkpointed step environments of the gradient kernels: the register
+// contents column by column (element (a, k) at k * 16 + a), so each store or
+// load instruction covers four whole rows of 16 elements.
+__device__ __forceinline__ void env_load_ck(const double2 *p, int lane, Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const double2 v = p[kdx(g, t) * 16 + a];
+    e.re[t] = v.x;
+    e.im[t] = v.y;
+  }
+}
+
+__device__ __forceinline__ void env_store_ck(double2 *p, int lane, const Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) p[kdx(g, t) * 16 + a] = make_double2(e.re[t], e.im[t]);
+}
+
 // Forest environments of Hermitian Pauli words are Hermitian, and so are
 // their adjoints.  The reverse-gradient kernels' hand-off slots keep the upper
 // triangle: element (a, k), a <= k, at a (31 - a) / 2 + k (136 entries, 2176
