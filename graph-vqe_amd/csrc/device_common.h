@@ -105,10 +105,12 @@ __device__ __forceinline__ void env_store(double2 *p, int lane, const Env &e) {
   for (int t = 0; t < 4; ++t) q[kdx(g, t)] = make_double2(e.re[t], e.im[t]);
 }
 
-// Checkpointed step environments of the gradient kernels: the register
-// contents column by column (element (a, k) at k * 16 + a), so each store or
-// load instruction covers four whole rows of 16 elements.
-__device__ __forceinline__ void env_load_ck(const double2 *p, int lane, Env &e) {
+// Column-major slots (element (a, k) at k * 16 + a): each store or load
+// instruction covers four whole rows of 16 elements, where the row-major form
+// touches sixteen.  Used for slots that only the chain kernels write and read
+// (chain hand-off spills, suffix stores and leaves of the complex energy
+// forests, the gradient's checkpointed step environments).
+__device__ __forceinline__ void env_load_cm(const double2 *p, int lane, Env &e) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -118,7 +120,7 @@ __device__ __forceinline__ void env_load_ck(const double2 *p, int lane, Env &e) 
   }
 }
 
-__device__ __forceinline__ void env_store_ck(double2 *p, int lane, const Env &e) {
+__device__ __forceinline__ void env_store_cm(double2 *p, int lane, const Env &e) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) p[kdx(g, t) * 16 + a] = make_double2(e.re[t], e.im[t]);

@@ -80,7 +80,7 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
   // (left of site 0 / right of site n-1); every other chain starts from a
   // spilled node
   Env e;
-  if (ch.src >= 0) env_load(spill + (size_t)ch.src * GV_ENV, lane, e);
+  if (ch.src >= 0) env_load_cm(spill + (size_t)ch.src * GV_ENV, lane, e);
   else env_unit(lane, e);
   double acc = 0.0;  // energy mode: this lane's share of sum_k c_k <P_k>
   StepDev sd = uload(P.steps + ch.step0);
@@ -123,17 +123,17 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
         else if (cd.kind == CK_Z) { ce.re[v] = yar[v] - ybr[v]; ce.im[v] = yai[v] - ybi[v]; }
         else { ce.re[v] = yai[v] - ybi[v]; ce.im[v] = -yar[v] + ybr[v]; }  // -i Y01 + i Y10
       }
-      if (cd.store >= 0) env_store(spill + (size_t)cd.store * GV_ENV, lane, ce);
+      if (cd.store >= 0) env_store_cm(spill + (size_t)cd.store * GV_ENV, lane, ce);
       // <P> = Tr(E_left R_right) = sum E[b][b'] Rt[b][b'] with the term's
       // suffix-forest environment (or the identity right environment);
       // nodes with many term ends leave them to k_leaf_terms
-      if (cd.defer >= 0) env_store(leafs + (inst.leaf_off + cd.defer) * GV_ENV, lane, ce);
+      if (cd.defer >= 0) env_store_cm(leafs + (inst.leaf_off + cd.defer) * GV_ENV, lane, ce);
       const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
 #pragma unroll 2
       for (int t = 0; t < nterm_inline; ++t) {
         const TermEnd te = uload(P.term_list + cd.term0 + t);
         Env r;
-        if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+        if (te.rslot >= 0) env_load_cm(spill + (size_t)te.rslot * GV_ENV, lane, r);
         else env_unit(lane, r);  // term ends at site n-1
         double sr = 0.0, si = 0.0;
 #pragma unroll
@@ -151,7 +151,7 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
         }
       }
       if (cd.action == -2) cont = ce;
-      else if (cd.action >= 0) env_store(spill + (size_t)cd.action * GV_ENV, lane, ce);
+      else if (cd.action >= 0) env_store_cm(spill + (size_t)cd.action * GV_ENV, lane, ce);
     }
     }
     e = cont;
@@ -206,13 +206,13 @@ __device__ __forceinline__ void leaf_body(const int2 *__restrict__ items, int ni
   const LeafDev lf = uload(P.leaves + it.y);
   const double2 *spill = spills + inst.spill_off;
   Env e;
-  env_load(leafs + (inst.leaf_off + it.y) * GV_ENV, lane, e);
+  env_load_cm(leafs + (inst.leaf_off + it.y) * GV_ENV, lane, e);
   double acc = 0.0;
 #pragma unroll 4
   for (int t = 0; t < lf.nterm; ++t) {
     const TermEnd te = uload(P.term_list + lf.term0 + t);
     Env r;
-    if (te.rslot >= 0) env_load(spill + (size_t)te.rslot * GV_ENV, lane, r);
+    if (te.rslot >= 0) env_load_cm(spill + (size_t)te.rslot * GV_ENV, lane, r);
     else env_unit(lane, r);
     double sr = 0.0, si = 0.0;
 #pragma unroll

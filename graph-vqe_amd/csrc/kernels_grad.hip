@@ -161,7 +161,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     Frag f;
     frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, false);
     const int cs = uload(P.ck_slot + step);
-    if (cs >= 0) env_store_ck(es + (size_t)cs * GV_ENV, lane, e);
+    if (cs >= 0) env_store_cm(es + (size_t)cs * GV_ENV, lane, e);
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
     prod_x(e, f, 1, x1r, x1i);
@@ -341,11 +341,11 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
     Env E;
     const int cs = uload(P.ck_slot + step);
     if (cs >= 0) {
-      env_load_ck(es + (size_t)cs * GV_ENV, lane, E);
+      env_load_cm(es + (size_t)cs * GV_ENV, lane, E);
     } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
       int s0 = st;
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
-      env_load_ck(es + (size_t)uload(P.ck_slot + ch.step0 + s0) * GV_ENV, lane, E);
+      env_load_cm(es + (size_t)uload(P.ck_slot + ch.step0 + s0) * GV_ENV, lane, E);
       for (int q = s0; q < st; ++q) advance(P, sites, ch.step0 + q, lane, E);
     }
     Frag fa;  // E layout of M_i (A operand form)

@@ -52,16 +52,17 @@ __device__ __forceinline__ void renv_store(double *p, int lane, const REnv &e) {
   for (int t = 0; t < 4; ++t) p[a * 16 + kdx(g, t)] = e.v[t];
 }
 
-// Checkpointed step environments: the register contents column by column
-// (element (a, k) at k * 16 + a), so each store or load instruction covers
-// four whole 128-byte rows.  Only the gradient kernels read these slots.
-__device__ __forceinline__ void renv_load_ck(const double *p, int lane, REnv &e) {
+// Column-major slots (element (a, k) at k * 16 + a): each store or load
+// instruction covers four whole 128-byte rows, where the row-major form
+// touches sixteen.  Used for slots that only these kernels write and read
+// (the gradient's checkpointed step environments).
+__device__ __forceinline__ void renv_load_cm(const double *p, int lane, REnv &e) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) e.v[t] = p[kdx(g, t) * 16 + a];
 }
 
-__device__ __forceinline__ void renv_store_ck(double *p, int lane, const REnv &e) {
+__device__ __forceinline__ void renv_store_cm(double *p, int lane, const REnv &e) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) p[kdx(g, t) * 16 + a] = e.v[t];
@@ -76,6 +77,8 @@ __device__ __forceinline__ void renv_unit(int lane, REnv &e) {
 // kernels' slots (chain hand-off spills, suffix-forest stores, deferred prefix
 // leaves) keep the upper triangle: element (a, k), a <= k, at a (31 - a) / 2 + k, 136 doubles
 // (9 of the 16 cache lines of a full 2 KiB environment).  Loads mirror it.
+// (Column by column, k (k + 1) / 2 + a, makes the stores contiguous but its
+// index arithmetic spills the I/Z kernels: 2.03 -> 2.13 ms.)
 __device__ __forceinline__ int sym_idx(int a, int k) { return a <= k ? a * (31 - a) / 2 + k : k * (31 - k) / 2 + a; }
 
 __device__ __forceinline__ void renv_store_sym(double *p, int lane, const REnv &e) {
@@ -316,7 +319,7 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
     // checkpoint store first: the prefetch below is then the youngest memory
     // operation when this step waits for f
     const int cs = uload(P.ck_slot + ch.step0 + st);
-    if (cs >= 0) renv_store_ck(rslot(es, cs), lane, e);
+    if (cs >= 0) renv_store_cm(rslot(es, cs), lane, e);
     nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));  // unconditional prefetch, as above
     rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
@@ -400,11 +403,11 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     REnv E;
     const int cs = uload(P.ck_slot + step);
     if (cs >= 0) {
-      renv_load_ck(rslot(es, cs), lane, E);
+      renv_load_cm(rslot(es, cs), lane, E);
     } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
       int s0 = st;
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
-      renv_load_ck(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
+      renv_load_cm(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
       for (int q = s0; q < st; ++q) radvance(P, sites, ch.step0 + q, lane, E);
     }
     RFrag fa;  // M_i in the A-operand form
