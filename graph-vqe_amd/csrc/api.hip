@@ -469,16 +469,19 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     // real tape, no Y words: the forests and their adjoints in real arithmetic
     hip_check(kl_pack_real(B.n_site_total, stream, B.garena, B.rarena), "k_pack_real launch");
     B.mark(1, stream);
-    for (int g = 0; g < B.max_ggen; ++g) {
-      const int ni = B.gn_items[g];
+    // the energy launches, deferred term ends by the leaf kernel
+    for (int g = 0; g < B.max_gen; ++g) {
+      const int ni = B.n_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.rarena, B.n_site_total,
-                                B.spills, B.parts,
-                                B.estore, B.gleafs),
+      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+                                B.spills, B.parts, B.estore, B.gleafs, B.leafs),
                 "k_rchains_gfwd launch");
     }
+    hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills, nullptr,
+                             B.parts),
+              "k_rleaf_terms launch");
     B.mark(2, stream);
-    hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 0), "k_reduce launch");
+    hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 1), "k_reduce launch");
     B.mark(3, stream);
     for (int g = B.max_ggen - 1; g >= 0; --g) {
       const int ni = B.gn_items[g];

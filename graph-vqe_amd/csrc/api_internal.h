@@ -118,7 +118,7 @@ hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const In
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
-                           double2 *estore, double2 *gleafs);
+                           double2 *estore, double2 *gleafs, double2 *leafs);
 hipError_t kl_rchains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
                           const ProbDev *probs, const double *rgarena, long long nsite, const double2 *spills,
                           double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot);

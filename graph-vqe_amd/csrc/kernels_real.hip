@@ -289,11 +289,15 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
 
 // Forward: energy chains keeping the checkpointed step environments and the
 // prefix leaves the backward pass reads (k_chains_gfwd in real arithmetic).
+// It runs the energy launches (shallow prefix depths beside the suffix
+// forest): deferred term ends go to leaf slots for k_rleaf_energy, exactly as
+// in the energy pass.  The backward pass reads only what this pass stores, so
+// it keeps its own generation order.
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
                const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
-               double2 *__restrict__ estore, double2 *__restrict__ gleafs) {
+               double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
   const int lane = threadIdx.x & 63;
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
@@ -337,7 +341,9 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
         for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
         if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
         if (cd.gleaf >= 0) renv_store_sym(rslot(gl, cd.gleaf), lane, ce);
-        for (int t = 0; t < cd.nterm; ++t) {
+        if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
+        const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
+        for (int t = 0; t < nterm_inline; ++t) {
           const TermEnd te = uload(P.term_list + cd.term0 + t);
           REnv r;
           if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
@@ -630,9 +636,9 @@ hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *i
 
 hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
-                           double2 *estore, double2 *gleafs) {
+                           double2 *estore, double2 *gleafs, double2 *leafs) {
   hipLaunchKernelGGL(k_rchains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
-                     rgarena, rgarena + nsite, spills, parts, estore, gleafs);
+                     rgarena, rgarena + nsite, spills, parts, estore, gleafs, leafs);
   return hipGetLastError();
 }
 
