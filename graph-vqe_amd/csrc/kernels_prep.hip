@@ -223,9 +223,21 @@ __device__ __forceinline__ bool qr_step(double2 (&col)[32], double2 *vb, int j, 
     double dr[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int r = R0; r < 32; ++r) dr[r & 3] = fma(v[r - R0], col[r].x, dr[r & 3]);
-    const double dx = tau * ((dr[0] + dr[1]) + (dr[2] + dr[3]));
+    // 1 / (alpha (alpha + |x_j|)) by the hardware reciprocal and two Newton
+    // steps (within an ulp of the quotient; the division's scaling sequence is
+    // only needed near the ends of the exponent range)
+    const double den = alpha * (alpha + axj);
+    double tr;
+    if (__builtin_expect(den > 1e-280 && den < 1e280, 1)) {
+      tr = __builtin_amdgcn_rcp(den);
+      tr = fma(tr, fma(-den, tr, 1.0), tr);
+      tr = fma(tr, fma(-den, tr, 1.0), tr);
+    } else {
+      tr = 1.0 / den;
+    }
+    const double dx = tr * ((dr[0] + dr[1]) + (dr[2] + dr[3]));
 #pragma unroll
-    for (int r = R0; r < 32; ++r) col[r].x -= v[r - R0] * dx;
+    for (int r = R0; r < 32; ++r) col[r].x = __builtin_fma(-v[r - R0], dx, col[r].x);
     return true;
   }
   if (lane == bi) {
