@@ -299,7 +299,7 @@ def bench_uneven(args):
     return {"partitions": len(parts), "qubits": f"{min(ns)}-{max(ns)}",
             "shots_per_partition": shots, "graph_ms": round(graph_ms, 3), "sequential_ms": round(seq_ms, 3),
             "speedup": round(seq_ms / graph_ms, 2), "energies_match_sequential_bitwise": same,
-            "autotuned_launch": {"waves_per_group": best[0], "order": ["instance-major", "longest-first"][best[1]]},
+            "autotuned_launch": {"waves_per_group": best[0], "order": ["instance-major", "longest-first-per-xcd"][best[1]]},
             "tuning_ms": {f"{k[0]}w/{'len' if k[1] else 'inst'}": round(v, 3) for k, v in table.items()},
             "shots_per_s": round(len(parts) * shots / (graph_ms * 1e-3), 1)}
 

@@ -74,6 +74,22 @@ void Batch::collect() {
   ev_used = 0;
 }
 
+// Longest chains first within each XCD's share of the blocks.  The chain
+// kernels give XCD x (block b runs on XCD b % 8) a contiguous range of
+// logical blocks of `wpb` items, so sorting inside each range keeps every
+// item on the XCD, and in the L2, it has in instance-major order while the
+// long chains start first and the short ones fill the launch's tail.
+static void xcd_longest_first(const Batch &B, std::vector<int2> &items, int wpb) {
+  const int n = (int)items.size(), nb = (n + wpb - 1) / wpb, q8 = nb >> 3, r8 = nb & 7;
+  auto len = [&](const int2 &a) { return B.probs[B.inst[a.x].prob]->chains[a.y].nsteps; };
+  for (int x = 0; x < 8; ++x) {
+    const int b0 = x * q8 + std::min(x, r8), b1 = b0 + q8 + (x < r8 ? 1 : 0);
+    const int i0 = std::min(n, b0 * wpb), i1 = std::min(n, b1 * wpb);
+    std::stable_sort(items.begin() + i0, items.begin() + i1,
+                     [&](const int2 &a, const int2 &b) { return len(a) > len(b); });
+  }
+}
+
 Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_problems, int device) {
   if (n_problems <= 0) throw std::runtime_error("empty batch");
   int ndev = 0;
@@ -212,7 +228,8 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->leafs = upload<double2>(B->owned, nullptr, (size_t)leaf * GV_ENV);
   B->leaf_items = upload(B->owned, leaf_items.data(), leaf_items.size());
   B->n_leaf_items = (int)leaf_items.size();
-  // chain work items per generation, instance-major
+  // chain work items per generation, instance-major, and the same items
+  // longest chains first within each XCD's blocks
   for (int g = 0; g < B->max_gen; ++g) {
     std::vector<int2> items;
     for (size_t i = 0; i < B->inst.size(); ++i) {
@@ -222,12 +239,8 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     }
     B->items.push_back(upload(B->owned, items.data(), items.size()));
     B->n_items.push_back((int)items.size());
-    // the same work items, longest chains first across all instances
-    std::stable_sort(items.begin(), items.end(), [&](const int2 &a, const int2 &b) {
-      const int la = B->probs[B->inst[a.x].prob]->chains[a.y].nsteps;
-      const int lb = B->probs[B->inst[b.x].prob]->chains[b.y].nsteps;
-      return la > lb;
-    });
+    B->items_host.push_back(items);
+    xcd_longest_first(*B, items, B->wpb);
     B->items_len.push_back(upload(B->owned, items.data(), items.size()));
   }
   for (int g = 0; g < B->max_ggen; ++g) {
@@ -240,6 +253,9 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     }
     B->gitems.push_back(upload(B->owned, items.data(), items.size()));
     B->gn_items.push_back((int)items.size());
+    B->gitems_host.push_back(items);
+    xcd_longest_first(*B, items, B->wpb);
+    B->gitems_len.push_back(upload(B->owned, items.data(), items.size()));
   }
   B->set_checkpoint(1);
   return B.release();
@@ -486,7 +502,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     for (int g = B.max_ggen - 1; g >= 0; --g) {
       const int ni = B.gn_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                B.spills, B.adjspill,
                                B.estore, B.gleafs, B.aslot),
                 "k_rchains_bwd launch");
@@ -506,7 +522,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   for (int g = 0; g < B.max_ggen; ++g) {
     const int ni = B.gn_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+    hip_check(kl_chains_gfwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
                              B.parts, B.estore,
                              B.gleafs),
               "k_chains_gfwd launch");
@@ -517,7 +533,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   for (int g = B.max_ggen - 1; g >= 0; --g) {
     const int ni = B.gn_items[g];
     if (ni == 0) continue;
-    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.gitems[g], B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
                             B.adjspill, B.estore,
                             B.gleafs, B.aslot),
               "k_chains_bwd launch");
@@ -759,6 +775,22 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
     if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
       throw std::runtime_error("waves per workgroup must be 1, 2 or 4");
     if (order != 0 && order != 1) throw std::runtime_error("order must be 0 (instance-major) or 1 (longest first)");
+    if (waves_per_group != B.wpb) {  // the XCD ranges depend on the workgroup size
+      hip_check(hipSetDevice(B.device), "hipSetDevice");
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      for (size_t g = 0; g < B.items_host.size(); ++g) {
+        std::vector<int2> items = B.items_host[g];
+        xcd_longest_first(B, items, waves_per_group);
+        hip_check(hipMemcpy(B.items_len[g], items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+      }
+      for (size_t g = 0; g < B.gitems_host.size(); ++g) {
+        std::vector<int2> items = B.gitems_host[g];
+        xcd_longest_first(B, items, waves_per_group);
+        hip_check(hipMemcpy(B.gitems_len[g], items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+      }
+    }
     B.wpb = waves_per_group;
     B.order = order;
   })

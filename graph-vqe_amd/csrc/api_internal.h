@@ -38,16 +38,19 @@ struct Batch {
   int2 *leaf_items = nullptr;     // (instance, leaf) work items
   int n_leaf_items = 0;
   std::vector<int2 *> items;      // per generation, instance-major
-  std::vector<int2 *> items_len;  // per generation, longest chains first across instances
+  std::vector<int2 *> items_len;  // per generation, longest chains first within each XCD's blocks (for wpb)
+  std::vector<std::vector<int2>> items_host;  // instance-major, host copy
   std::vector<int> n_items;
   int max_gen = 0;
   // reverse-gradient generations (suffix forest, then prefix forest)
   std::vector<int2 *> gitems;
+  std::vector<int2 *> gitems_len;
+  std::vector<std::vector<int2>> gitems_host;
   std::vector<int> gn_items;
   int max_ggen = 0;
   // launch configuration of the chain kernels (set by the autotuner)
   int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
-  int order = 0;  // 0: instance-major items, 1: longest chains first
+  int order = 1;  // 0: instance-major items, 1: longest chains first within each XCD's share
   int prep_mode = 0;  // 0: gates + SVDs (k_prep), 1: exact bond doubling (k_build_local)
   // real-arithmetic energies: every problem has real gates and no Y words
   bool real = false;
@@ -55,6 +58,7 @@ struct Batch {
   int arith = 0;              // 0: real path when `real`, 1: complex path always
   double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
+  const int2 *ggen_items(int g) const { return order ? gitems_len[g] : gitems[g]; }
   // captured evaluate + sample sequence
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;

@@ -3,8 +3,9 @@
 The forest-chain kernels run one wavefront per (instance, chain).  Two knobs
 change only how that work is packed onto the chip, never the results: the
 wavefronts per workgroup (1, 2 or 4) and the work-item order (instance-major,
-or the longest chains first across all instances, which shortens a
-generation's tail when partitions of different sizes share a launch).  The
+or the longest chains first within each XCD's share of the instance-major
+items, which shortens a generation's tail while every XCD keeps the same
+instances in its L2).  The
 tuner times every combination on the batch's own workload with HIP events on
 the launch stream and keeps the fastest.  Choices are cached per batch
 signature, in memory and optionally in a JSON file (GVMPS_TUNE_CACHE).

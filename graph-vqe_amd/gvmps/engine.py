@@ -154,8 +154,10 @@ class Batch:
                                            ctypes.c_void_p(g.data_ptr()), _stream_ptr(t)))
         return out, g[:self.n_params_total]
 
-    def set_launch(self, waves_per_group=4, order=0):
-        """Chain-kernel launch shape (results do not depend on it)."""
+    def set_launch(self, waves_per_group=4, order=1):
+        """Chain-kernel launch shape (results do not depend on it): waves per
+        workgroup, and item order 0 (instance-major) or 1 (longest chains
+        first within each XCD's share of the instance-major items)."""
         check(lib().gvmps_batch_set_launch(self._h, int(waves_per_group), int(order)))
         self.launch = (int(waves_per_group), int(order))
 

@@ -144,8 +144,10 @@ int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *ene
 int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride);
 
 /* Launch configuration of the forest-chain kernels: wavefronts per workgroup
- * (1, 2 or 4) and work-item order (0 instance-major, 1 longest chains first
- * across instances).  Results do not depend on it.  No reference counterpart
+ * (1, 2 or 4) and work-item order (0 instance-major, 1 (default) longest
+ * chains first within each XCD's share of the instance-major items, so each
+ * XCD keeps the same instances in its L2).  Results do not depend on it.
+ * No reference counterpart
  * (the reference launches fixed shapes, sharded_mps.cuh:670-684); set by the
  * autotuner (gvmps/autotune.py). */
 int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order);
