@@ -79,15 +79,19 @@ void Batch::collect() {
 // logical blocks of `wpb` items, so sorting inside each range keeps every
 // item on the XCD, and in the L2, it has in instance-major order while the
 // long chains start first and the short ones fill the launch's tail.
-static void xcd_longest_first(const Batch &B, std::vector<int2> &items, int wpb) {
+template <typename Len>
+static void xcd_longest_first(std::vector<int2> &items, int wpb, Len len) {
   const int n = (int)items.size(), nb = (n + wpb - 1) / wpb, q8 = nb >> 3, r8 = nb & 7;
-  auto len = [&](const int2 &a) { return B.probs[B.inst[a.x].prob]->chains[a.y].nsteps; };
   for (int x = 0; x < 8; ++x) {
     const int b0 = x * q8 + std::min(x, r8), b1 = b0 + q8 + (x < r8 ? 1 : 0);
     const int i0 = std::min(n, b0 * wpb), i1 = std::min(n, b1 * wpb);
     std::stable_sort(items.begin() + i0, items.begin() + i1,
                      [&](const int2 &a, const int2 &b) { return len(a) > len(b); });
   }
+}
+
+static auto chain_len(const Batch &B) {
+  return [&B](const int2 &a) { return B.probs[B.inst[a.x].prob]->chains[a.y].nsteps; };
 }
 
 Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_problems, int device) {
@@ -226,6 +230,11 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->seeds = upload<unsigned>(B->owned, nullptr, B->inst.size());
   B->parts = upload<double>(B->owned, nullptr, part);
   B->leafs = upload<double2>(B->owned, nullptr, (size_t)leaf * GV_ENV);
+  // the leaf kernels use the XCD-aware block order of the chain kernels (4
+  // waves per block): each instance's leaves run on the XCD its chains used,
+  // the leaves with most term ends first
+  xcd_longest_first(leaf_items, 4,
+                    [&](const int2 &a) { return B->probs[B->inst[a.x].prob]->leaves[a.y].nterm; });
   B->leaf_items = upload(B->owned, leaf_items.data(), leaf_items.size());
   B->n_leaf_items = (int)leaf_items.size();
   // chain work items per generation, instance-major, and the same items
@@ -240,7 +249,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     B->items.push_back(upload(B->owned, items.data(), items.size()));
     B->n_items.push_back((int)items.size());
     B->items_host.push_back(items);
-    xcd_longest_first(*B, items, B->wpb);
+    xcd_longest_first(items, B->wpb, chain_len(*B));
     B->items_len.push_back(upload(B->owned, items.data(), items.size()));
   }
   for (int g = 0; g < B->max_ggen; ++g) {
@@ -254,7 +263,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     B->gitems.push_back(upload(B->owned, items.data(), items.size()));
     B->gn_items.push_back((int)items.size());
     B->gitems_host.push_back(items);
-    xcd_longest_first(*B, items, B->wpb);
+    xcd_longest_first(items, B->wpb, chain_len(*B));
     B->gitems_len.push_back(upload(B->owned, items.data(), items.size()));
   }
   B->set_checkpoint(1);
@@ -780,13 +789,13 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
       hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       for (size_t g = 0; g < B.items_host.size(); ++g) {
         std::vector<int2> items = B.items_host[g];
-        xcd_longest_first(B, items, waves_per_group);
+        xcd_longest_first(items, waves_per_group, chain_len(B));
         hip_check(hipMemcpy(B.items_len[g], items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice),
                   "hipMemcpy");
       }
       for (size_t g = 0; g < B.gitems_host.size(); ++g) {
         std::vector<int2> items = B.gitems_host[g];
-        xcd_longest_first(B, items, waves_per_group);
+        xcd_longest_first(items, waves_per_group, chain_len(B));
         hip_check(hipMemcpy(B.gitems_len[g], items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice),
                   "hipMemcpy");
       }

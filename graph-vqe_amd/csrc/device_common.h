@@ -42,6 +42,16 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 // VGPRs and all control flow on it runs under exec masks.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// This wave's work item under the XCD-aware block order: block b runs on XCD
+// b % 8, and XCD x takes a contiguous range of logical blocks, so
+// consecutive items (one instance's chains or leaves) share an XCD and its L2.
+__device__ __forceinline__ int xcd_item() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
+  const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
+  return logical * (int)(blockDim.x >> 6) + wave_id();
+}
+
 // Wave-uniform load of read-only problem data (plans, work lists, tables the
 // host uploads before the launch) through the constant address space: scalar
 // loads into SGPRs.  A plain load of the same data becomes a per-lane vector

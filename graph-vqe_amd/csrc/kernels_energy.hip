@@ -198,7 +198,7 @@ __device__ __forceinline__ void leaf_body(const int2 *__restrict__ items, int ni
                                           const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
                                           double2 *__restrict__ terms, double *__restrict__ parts) {
   const int lane = threadIdx.x & 63;
-  const int w = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
+  const int w = xcd_item();
   if (w >= nitems) return;
   const int2 it = uload(items + w);
   const InstDev inst = uload(insts + it.x);
