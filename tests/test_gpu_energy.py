@@ -194,6 +194,42 @@ def test_spsa_40q_config_parity_and_batch_invariance(cuda):
     assert np.all(np.abs((tv * c).sum(axis=1).real - e_terms) <= 1e-10 * np.abs(e_terms))
 
 
+@pytest.mark.skip(reason="launch order 2 (one persistent workgroup per instance, chains started as their "
+                         "environments are published through LDS bits) did not finish within the test's 120 s on "
+                         "its first GPU run; no cause was found by reading the kernel or by a CPU emulation of its "
+                         "schedule, so the path is not in the library (set_launch rejects order 2)")
+def test_persistent_instance_forests_bitwise(cuda):
+    """Launch order 2 (one persistent workgroup per instance, chains started as
+    their environments are stored) gives bitwise the energies of the
+    generation launches (orders 0 and 1), on the 40-qubit workload and on a
+    small problem with inline and deferred term ends."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    assert p.info()['iz_forest'] == 1
+    print('case: 40q SPSA batch of 256, orders 0/1/2', flush=True)
+    b = Batch([p], [256])
+    ref = b.energies(w['x']).cpu().numpy()
+    for wpb, order in ((4, 1), (4, 2), (2, 2), (4, 2)):
+        b.set_launch(wpb, order)
+        assert np.array_equal(b.energies(w['x']).cpu().numpy(), ref), (wpb, order)
+    r = np.random.default_rng(11)
+    n = 14
+    ops, P = workloads.su2_ansatz(n, 3, ('ry',), 'linear')
+    words = [''.join(r.choice(list('IIIZ'), n)) for _ in range(300)] + ['I' * n]
+    coeffs = r.normal(size=len(words))
+    q = Problem(n, ops, P, words, coeffs, max_bond=16, abs_cut=0, rel_cut=0)
+    assert q.info()['iz_forest'] == 1 and q.info()['real_arith'] == 1
+    X = r.uniform(-np.pi, np.pi, (64, P))
+    print(f'case: {n}q, 300 Z-strings, batch 64, orders 0/2', flush=True)
+    bq = Batch([q], [64])
+    bq.set_launch(4, 0)
+    e0 = bq.energies(X).cpu().numpy()
+    bq.set_launch(4, 2)
+    assert np.array_equal(bq.energies(X).cpu().numpy(), e0)
+    d = dense.Dense(n).run(tape_with_values(ops, X[0]))
+    assert rel_close(e0[0], d.energy(words, coeffs))
+
+
 def test_bond_doubling_preparation_matches_svd(cuda):
     """Preparation mode 'doubling' (exact CNOT bond doubling, no SVD) gives the
     same energies (1e-10 relative) and the same samples as the SVD path."""
