@@ -2,9 +2,9 @@
 //
 //   k_envs    one wavefront per (instance, direction): identity left
 //             environments L_s and transposed right environments Rt_s.
-//   k_chains  one wavefront per (instance, chain) of the prefix forest: the
-//             environment stays in MFMA registers along the chain; children
-//             that start other chains are spilled to HBM.
+//   k_chains  one wavefront per (instance, chain) of the prefix and suffix
+//             forests: the environment stays in MFMA registers along the
+//             chain; children that start other chains are spilled to HBM.
 //   k_reduce  one workgroup per instance: E = Re sum_k c_k <P_k>, compensated
 //             and in a fixed order.
 #include "device_common.h"

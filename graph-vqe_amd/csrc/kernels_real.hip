@@ -11,13 +11,13 @@
 //   real path                   X = E M   (one MFMA product per complex product)
 //
 // 16 MFMAs per I/Z step instead of 48, 4 KiB site slots instead of 8 KiB, and
-// environments of 2 KiB (stored in the first half of the complex 4 KiB spill
-// and leaf slots).  The packed real arena holds every site twice: as A_i and
-// as A_i^T (suffix-forest chains read the transposed copy), so every fragment
-// load is a row load with 128-byte lines per 16 lanes.  The chain/forest
-// schedule, work items, term ends and
-// partial-energy layout are those of kernels_energy.hip, so k_reduce_energy /
-// k_reduce combine the results unchanged.
+// symmetric environments kept as packed upper triangles (1088 bytes) in the
+// 4 KiB spill, store and leaf slots.  The packed real arena holds every site
+// twice: as A_i and as A_i^T (suffix-forest chains read the transposed copy),
+// so every fragment load is a row load with 128-byte lines per 16 lanes.  The
+// chain/forest schedule, work items, term ends and partial-energy layout are
+// those of kernels_energy.hip, so k_reduce_energy / k_reduce combine the
+// results unchanged.
 #include "device_common.h"
 
 using namespace gv;
