@@ -369,11 +369,11 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
     if (terms)
-      hip_check(kl_chains(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total, B.leafs,
+      hip_check(kl_chains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total, B.leafs,
                           B.spills, B.terms),
                 "k_chains launch");
     else
-      hip_check(kl_chains_energy(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total,
+      hip_check(kl_chains_energy(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total,
                                  B.leafs, B.spills,
                                  B.parts),
                 "k_chains_energy launch");

@@ -101,13 +101,13 @@ hipError_t kl_prep(int ninst, bool real, hipStream_t s, const InstDev *insts, co
                    const double *params, double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
 hipError_t kl_transpose_sites(long long n, hipStream_t s, double2 *arena);
-hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_chains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                      const double2 *arena, long long nsite, double2 *leafs, double2 *spills, double2 *terms);
 hipError_t kl_leaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                          const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_reduce(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs, double2 *terms,
                      double *energies);
-hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_chains_energy(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                             const ProbDev *probs, const double2 *arena, long long nsite, double2 *leafs,
                             double2 *spills, double *parts);
 hipError_t kl_reduce_energy(int ninst, hipStream_t s, const InstDev *insts, const ProbDev *probs,

@@ -33,6 +33,10 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 // waves per SIMD (80 registers; measured 2% faster than five, 4% than four).
 #define GV_IZ_WAVES __attribute__((amdgpu_waves_per_eu(6, 6)))
 // The per-term form keeps more live state: five waves.
+// Complex chains of I/Z-only forests (k_chains*_iz).
+#ifndef GV_CIZ_WAVES
+#define GV_CIZ_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+#endif
 #define GV_FIVE_WAVES __attribute__((amdgpu_waves_per_eu(5, 5)))
 #define GV_IZ_TERM_UNROLL 1
 

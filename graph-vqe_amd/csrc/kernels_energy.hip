@@ -52,7 +52,7 @@ k_envs(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict__
 // wave.  TERMS: write every <P_k> (observe's per-term data); otherwise each
 // lane accumulates its share of sum_k c_k <P_k> and the chain writes one
 // partial energy (no per-term wave reductions).
-template <bool TERMS>
+template <bool TERMS, bool IZ>
 __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int nitems,
                                             const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
                                             const double2 *__restrict__ arena, long long nsite,
@@ -102,8 +102,8 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
     // pass 1: X/Y children from Y01 = A_0^H E A_1 and Y10 = A_1^H E A_0
     // (one pair of products live at a time)
 #pragma nounroll
-    for (int pass = 0; pass < 2; ++pass) {
-    if (!(sd.need & (1 << pass))) continue;
+    for (int pass = 0; pass < (IZ ? 1 : 2); ++pass) {
+    if (!IZ && !(sd.need & (1 << pass))) continue;
     d4 yar, yai, ybr, ybi;
     if (pass == 0) {
       prod_y(x0r, x0i, f, 0, yar, yai);
@@ -114,7 +114,7 @@ __device__ __forceinline__ void chains_body(const int2 *__restrict__ items, int 
     }
     for (int c = 0; c < sd.nchild; ++c) {
       const ChildDev cd = uload(P.children + sd.child0 + c);
-      if ((cd.kind == CK_X || cd.kind == CK_Y) != (pass == 1)) continue;
+      if (!IZ && (cd.kind == CK_X || cd.kind == CK_Y) != (pass == 1)) continue;
       Env ce;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -168,14 +168,31 @@ extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
          const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, long long nsite,
          double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  chains_body<true>(items, nitems, insts, probs, arena, nsite, leafs, spills, terms, nullptr);
+  chains_body<true, false>(items, nitems, insts, probs, arena, nsite, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_TWO_WAVES
 k_chains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                 const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, long long nsite,
                 double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
-  chains_body<false>(items, nitems, insts, probs, arena, nsite, leafs, spills, nullptr, parts);
+  chains_body<false, false>(items, nitems, insts, probs, arena, nsite, leafs, spills, nullptr, parts);
+}
+
+// Forests whose steps have I/Z children only (no X or Y words): one product
+// pass, and the site fragment and X_i products die once Y00 and Y11 are
+// formed.  Same arithmetic per instance.
+extern "C" __global__ void __launch_bounds__(256) GV_CIZ_WAVES
+k_chains_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+            const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, long long nsite,
+            double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  chains_body<true, true>(items, nitems, insts, probs, arena, nsite, leafs, spills, terms, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_CIZ_WAVES
+k_chains_energy_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                   const ProbDev *__restrict__ probs, const double2 *__restrict__ arena, long long nsite,
+                   double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
+  chains_body<false, true>(items, nitems, insts, probs, arena, nsite, leafs, spills, nullptr, parts);
 }
 
 // Transposed copy of the site arena: slot s of the second half holds A_i^T
@@ -338,10 +355,11 @@ hipError_t kl_transpose_sites(long long n, hipStream_t s, double2 *arena) {
   return hipGetLastError();
 }
 
-hipError_t kl_chains(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                     const double2 *arena, long long nsite, double2 *leafs, double2 *spills, double2 *terms) {
-  hipLaunchKernelGGL(k_chains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
-                     nsite, leafs, spills, terms);
+hipError_t kl_chains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
+                     const ProbDev *probs, const double2 *arena, long long nsite, double2 *leafs, double2 *spills,
+                     double2 *terms) {
+  hipLaunchKernelGGL(iz ? k_chains_iz : k_chains, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items,
+                     nitems, insts, probs, arena, nsite, leafs, spills, terms);
   return hipGetLastError();
 }
 
@@ -357,11 +375,11 @@ hipError_t kl_leaf_terms(int nitems, hipStream_t s, const int2 *items, const Ins
   return hipGetLastError();
 }
 
-hipError_t kl_chains_energy(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_chains_energy(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                             const ProbDev *probs, const double2 *arena, long long nsite, double2 *leafs,
                             double2 *spills, double *parts) {
-  hipLaunchKernelGGL(k_chains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs, arena,
-                     nsite, leafs, spills, parts);
+  hipLaunchKernelGGL(iz ? k_chains_energy_iz : k_chains_energy, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0,
+                     s, items, nitems, insts, probs, arena, nsite, leafs, spills, parts);
   return hipGetLastError();
 }
 
