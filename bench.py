@@ -162,7 +162,8 @@ def main():
         complex_leg = {"evals_per_s": round(cx_rate, 2), "forest_chains_ms": round(cx_chains, 3),
                        "achieved_tflops": round(cx_flops, 3), "frac": round(cx_flops / PEAK_FP64_TFLOPS, 4),
                        "max_rel_diff_vs_real": float(torch.max(torch.abs(e_cx - e) / torch.clamp(torch.abs(e), min=1.0))),
-                       "kernel": "k_chains_energy + k_leaf_energy (complex128, Gauss-form MFMA products)"}
+                       "kernel": ("k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy")
+                                 + " + k_leaf_energy (complex128, Gauss-form MFMA products)"}
         batch.set_arith('auto')
         batch.set_timing(False)
 
@@ -209,7 +210,8 @@ def main():
                      "traffic": measured_traffic(),
                      "traffic_basis": "HBM bytes per step (all generations), rocprofv3 PMC, copy-calibrated",
                      "kernel": (("k_rchains_energy_iz (real arithmetic, I/Z-only forest)" if info['iz_forest']
-                                 else "k_rchains_energy (real arithmetic)") if real else "k_chains_energy")
+                                 else "k_rchains_energy (real arithmetic)") if real
+                                else "k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy")
                                + " (all generations of one step) + leaf term-end kernel",
                      "algorithmic_flops_per_step": node_flops * B,
                      "executed_tflops": round(executed, 3)},
