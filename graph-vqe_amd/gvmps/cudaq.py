@@ -324,6 +324,44 @@ class _Spin:
 spin = _Spin()
 
 
+def qiskit_sparsepauliop_to_cudaq_spin(op, reverse_qubit_order=True, num_qubits=None):
+    """SpinOperator of a Qiskit ``SparsePauliOp`` (Qiskit_VQE/cudaq.py:25-47).
+
+    ``op`` is anything with ``to_list()`` (and ``num_qubits``), such as a
+    SparsePauliOp, or a plain list of ``(label, coeff)`` pairs, which is what
+    ``SparsePauliOp.to_list()`` returns; qiskit itself is not needed.  As in
+    the reference:
+      * labels are Qiskit strings, leftmost character = highest qubit, so with
+        ``reverse_qubit_order`` character idx acts on qubit n-1-idx (n = the
+        operator's qubit count, else the label length); without it, on idx;
+      * zero coefficients are skipped and only the real part of the others is
+        kept;
+      * an all-I label adds a constant; repeated labels add up.
+    Labels must hold only I, X, Y, Z.
+    """
+    pairs = list(op.to_list()) if hasattr(op, 'to_list') else list(op)
+    if num_qubits is None:
+        num_qubits = getattr(op, 'num_qubits', None)
+    H = SpinOperator()
+    for label, coeff in pairs:
+        label = str(label)
+        n = len(label) if num_qubits is None else int(num_qubits)
+        if len(label) != n:
+            raise ValueError(f'Pauli label {label!r} has {len(label)} characters for {n} qubits')
+        if coeff == 0:
+            continue
+        term = SpinOperator({(): 1.0})
+        for idx, ch in enumerate(label):
+            q = (n - 1 - idx) if reverse_qubit_order else idx
+            if ch == 'I':
+                continue
+            if ch not in 'XYZ':
+                raise ValueError(f'Pauli label {label!r} has {ch!r}; expected I, X, Y or Z')
+            term = term * getattr(spin, ch.lower())(q)
+        H = H + float(complex(coeff).real) * term
+    return H
+
+
 class _Parallel:
     thread = 'thread'
     mpi = 'mpi'

@@ -135,3 +135,51 @@ def test_entangling_patterns_match_reference_definitions():
     assert workloads.entangling_pairs(n, 0, 'reverse_linear') == [(i, i + 1) for i in range(n - 2, -1, -1)]
     assert workloads.entangling_pairs(n, 1, 'pairwise') == [(1, 2), (3, 4)]
     assert workloads.entangling_pairs(n, 0, 'circular') == [(i, i + 1) for i in range(n - 1)] + [(n - 1, 0)]
+
+
+class _FakeSparsePauliOp:
+    """The two members of qiskit's SparsePauliOp the converter reads."""
+
+    def __init__(self, pairs, n):
+        self._pairs, self.num_qubits = pairs, n
+
+    def to_list(self):
+        return list(self._pairs)
+
+
+def test_sparsepauliop_conversion_order_and_coefficients():
+    """Qiskit_VQE/cudaq.py:25-47: Qiskit labels are big-endian (character idx
+    acts on qubit n-1-idx), zero coefficients are dropped, only real parts
+    are kept, repeated labels add up and an all-I label is a constant."""
+    pairs = [('ZIXY', 0.5 + 0.2j), ('IIII', 1.25), ('IZII', 0.0), ('XIIZ', -0.75), ('XIIZ', 0.25)]
+    H = cudaq.qiskit_sparsepauliop_to_cudaq_spin(_FakeSparsePauliOp(pairs, 4))
+    words, coeffs = H.words(4)
+    got = {w: c for w, c in zip(words, coeffs)}
+    assert got == {'YXIZ': 0.5, 'IIII': 1.25, 'ZIIX': -0.5}
+    # plain (label, coeff) lists, and the unreversed order
+    H2 = cudaq.qiskit_sparsepauliop_to_cudaq_spin(pairs, reverse_qubit_order=False)
+    words2, coeffs2 = H2.words(4)
+    assert {w: c for w, c in zip(words2, coeffs2)} == {'ZIXY': 0.5, 'IIII': 1.25, 'XIIZ': -0.5}
+    with pytest.raises(ValueError):
+        cudaq.qiskit_sparsepauliop_to_cudaq_spin([('ZZ', 1.0)], num_qubits=3)
+    with pytest.raises(ValueError):
+        cudaq.qiskit_sparsepauliop_to_cudaq_spin([('ZA', 1.0)])
+
+
+def test_sparsepauliop_conversion_energy_closed_form():
+    """After RY(t_q) on every qubit of |0...0>, <Z_q> = cos t_q, <X_q> = sin t_q
+    and <Y_q> = 0, so a converted operator's energy on the dense state must
+    equal the product formula read off the Qiskit labels (qubit n-1-idx for
+    character idx).  A reversed qubit order fails this."""
+    from oracle import dense
+    n = 5
+    t = np.random.default_rng(3).uniform(-np.pi, np.pi, n)
+    labels = ['ZIIIZ', 'IXIZI', 'YIIII', 'IIXXI', 'ZZZZZ', 'IIIII']
+    coeffs = [0.7, -1.3, 2.0, 0.4, -0.9, 0.35]
+    H = cudaq.qiskit_sparsepauliop_to_cudaq_spin(_FakeSparsePauliOp(list(zip(labels, coeffs)), n))
+    words, cw = H.words(n)
+    state = dense.Dense(n).run([('ry', (q,), t[q]) for q in range(n)])
+    single = {'I': lambda a: 1.0, 'Z': np.cos, 'X': np.sin, 'Y': lambda a: 0.0}
+    want = sum(c * np.prod([single[ch](t[n - 1 - idx]) for idx, ch in enumerate(lab)])
+               for lab, c in zip(labels, coeffs))
+    assert abs(state.energy(words, cw) - want) < 1e-12
