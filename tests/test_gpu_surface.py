@@ -187,3 +187,27 @@ def test_lbfgs_with_adjoint_descends(cuda):
     assert res.njev_reverse > 0
     res_c = optimize.cobyla_minimize(optimize.BatchedObjective(kernel, H), x0, rhobeg=0.5, maxiter=40)
     assert res_c.fun < e0
+
+
+def test_backend_metrics_wrapper_composes_with_observe(tmp_path):
+    """The evaluation recorder wraps this module's observe/observe_async the
+    way the campaign wraps CUDA-Q's (kernel, Hamiltonian, then parameters):
+    same energies as the unwrapped calls, one row per evaluation."""
+    import json
+    import types
+    from gvmps import readout
+    n = 6
+    kernel, ops, count = build(n, 2, ('ry',), 'linear', spread=True)
+    H, words, coeffs = hamiltonian(n, np.random.default_rng(5), 20)
+    x = np.random.default_rng(6).uniform(-1, 1, count).tolist()
+    ns = types.SimpleNamespace(observe=cudaq.observe, observe_async=cudaq.observe_async,
+                               sample=cudaq.sample, sample_async=cudaq.sample_async)
+    log = tmp_path / 'rows.jsonl'
+    readout.install_backend_metrics(ns, path=str(log))
+    e0 = cudaq.observe(kernel, H, *x).expectation()
+    e1 = ns.observe(kernel, H, *x).expectation()
+    e2 = ns.observe_async(kernel, H, *x, qpu_id=0).get().expectation()
+    assert e1 == e0 and e2 == e0
+    rows = [json.loads(l) for l in log.read_text().splitlines()]
+    assert [r['operation'] for r in rows] == ['observe', 'observe_async_until_get']
+    assert all(r['parameters'] == x and r['parameter_count'] == count for r in rows)

@@ -183,3 +183,54 @@ def test_sparsepauliop_conversion_energy_closed_form():
     want = sum(c * np.prod([single[ch](t[n - 1 - idx]) for idx, ch in enumerate(lab)])
                for lab, c in zip(labels, coeffs))
     assert abs(state.energy(words, cw) - want) < 1e-12
+
+
+def test_protein_probabilities_reverses_and_normalises():
+    """test/tools/readout.py:4-9: q0-left counts to q0-right probabilities."""
+    from gvmps import readout
+    assert readout.protein_probabilities({'0011': 600, '1000': 400}) == {'1100': 0.6, '0001': 0.4}
+    with pytest.raises(ValueError):
+        readout.protein_probabilities({'01': 0})
+
+
+def test_backend_metrics_rows(tmp_path, monkeypatch):
+    """Rows of the campaign's analytic and sampled recorders: one per call,
+    async calls logged once at their first get(), shots forced, parameters
+    after the kernel (sample) or the kernel and Hamiltonian (observe)."""
+    import types
+    from gvmps import readout
+    calls = []
+
+    class Fut:
+        def get(self):
+            return 'async'
+
+    def fake(name):
+        def f(*args, **kw):
+            calls.append((name, kw.get('shots_count')))
+            return Fut() if name.endswith('_async') else name
+        return f
+    mod = types.SimpleNamespace(**{n: fake(n) for n in ('observe', 'observe_async', 'sample', 'sample_async')})
+    monkeypatch.setenv('CUDAQ_MPS_MAX_BOND', '16')
+    log = tmp_path / 'an.jsonl'
+    assert readout.install_backend_metrics(mod, path=str(log)) == ['observe', 'observe_async']
+    assert mod.observe('k', 'H', 0.5, 1.5, shots_count=1000) == 'observe'
+    fut = mod.observe_async('k', 'H', [0.25, 0.75], qpu_id=0)
+    assert fut.get() == 'async' and fut.get() == 'async'
+    rows = [json.loads(l) for l in log.read_text().splitlines()]
+    assert [r['operation'] for r in rows] == ['observe', 'observe_async_until_get']
+    assert calls == [('observe', -1), ('observe_async', -1)]
+    assert rows[0]['parameters'] == [0.5, 1.5] and rows[1]['parameters'] == [0.25, 0.75]
+    assert all(r['shots'] == -1 and r['physical_shots'] == 0 and r['mps_max_bond'] == '16'
+               and r['target'] == 'tensornet-mps-mqpu-shots' for r in rows)
+    mod2 = types.SimpleNamespace(**{n: fake(n) for n in ('observe', 'observe_async', 'sample', 'sample_async')})
+    log2 = tmp_path / 'sh.jsonl'
+    readout.install_backend_metrics(mod2, path=str(log2), shots=1000)
+    mod2.sample('k', 0.1, 0.2)
+    mod2.sample_async('k', 0.3).get()
+    mod2.observe('k', 'H', 0.4)
+    rows = [json.loads(l) for l in log2.read_text().splitlines()]
+    assert [r['operation'] for r in rows] == ['sample', 'sample_async', 'observe']
+    assert [r['parameters'] for r in rows] == [[0.1, 0.2], [0.3], [0.4]]
+    assert all(r['shots'] == 1000 and r['physical_shots_per_circuit_or_term'] == 1000
+               and r['added_noise_model'] is False for r in rows)
