@@ -228,8 +228,8 @@ def main(argv=None):
     ap.add_argument('--epochs', type=int, default=300)
     ap.add_argument('--finetune', type=int, default=0, help='energy fine-tuning steps after pretraining')
     ap.add_argument('--seed', type=int, default=2026)
-    ap.add_argument('--out', default='gpurun_out/learned_warmstart.json')
-    ap.add_argument('--weights', default='gpurun_out/warmstart_gnn.npz')
+    ap.add_argument('--out', default='learned_warmstart.json')
+    ap.add_argument('--weights', default='warmstart_gnn.npz')
     args = ap.parse_args(argv)
     t0 = time.time()
     train_h = hamiltonians(args.train, args.nmin, args.nmax, args.tmin, args.tmax, args.seed)
