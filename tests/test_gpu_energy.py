@@ -157,6 +157,69 @@ def test_nonzero_cutoffs_match_oracle(cuda):
     assert rel_close(e.cpu().numpy()[0], eo, 1e-9)
 
 
+def truncation_margins(sim, max_bond, cut):
+    """(truncating gates, smallest relative gap between the last kept and the
+    first dropped singular value, smallest relative distance of any singular
+    value from the cutoff threshold) over the oracle's SVDs."""
+    trunc, gap, thr_margin = 0, np.inf, np.inf
+    for s in sim.mps.singular_log:
+        k = mps_ref.select_rank(s, max_bond, cut, cut)
+        thr = max(cut, cut * s[0])
+        if k < len(s):
+            trunc += 1
+            if s[k] > 1e-13 * s[0]:
+                gap = min(gap, (s[k - 1] - s[k]) / s[0])
+        if cut > 0:
+            thr_margin = min(thr_margin, np.min(np.abs(s - thr)) / thr)
+    return trunc, gap, thr_margin
+
+
+# (qubits, reps, rotations, pattern, max_bond, cutoff, seed, truncates): the
+# surface's default engine (max bond 64, cutoffs 1e-5) on a real RY + CX
+# campaign circuit (every singular value above the threshold); a 1e-3 cutoff
+# that drops singular values at 5-6 gates; 22-qubit circuits whose 32 x 32
+# Theta blocks are cut to chi = 16 at 26 gates, one real and one complex
+TRUNCATING = [(14, 4, ('ry',), 'linear', 64, 1e-5, 1, False),
+              (12, 4, ('ry',), 'linear', 64, 1e-3, 2, True),
+              (22, 6, ('ry',), 'linear', 16, 1e-9, 3, True),
+              (22, 6, ('ry', 'rz'), 'reverse_linear', 16, 1e-9, 4, True)]
+
+
+@pytest.mark.parametrize('case', TRUNCATING, ids=['14q-default-engine', '12q-cutoff-1e-3', '22q-real-chi16',
+                                                  '22q-complex-chi16'])
+def test_truncating_jacobi_svd_matches_oracle(cuda, case):
+    """Nonzero cutoffs send every two-site gate through the one-sided Jacobi
+    SVD and the cutoff rank rule (dist_svd.cuh:194-231); with chi = 16 the
+    32 x 32 Theta blocks are truncated.  The oracle (LAPACK SVD) must show
+    the truncations well separated (singular values away from the threshold,
+    kept and dropped values apart) so that both sides cut the same subspace;
+    energies then agree to 1e-9 relative on the real and the complex
+    kernels."""
+    n, reps, gates, pattern, max_bond, cut, seed, truncates = case
+    r = np.random.default_rng(seed)
+    ops, P = workloads.su2_ansatz(n, reps, gates, pattern)
+    words, coeffs = workloads.z_hamiltonian(n, 200, 4, r)
+    X = np.stack([r.uniform(-np.pi, np.pi, P) for _ in range(2)])
+    print(f'case: truncating SVD n={n} reps={reps} gates={gates} max_bond={max_bond} cut={cut}', flush=True)
+    p = Problem(n, ops, P, words, coeffs, max_bond=max_bond, abs_cut=cut, rel_cut=cut)
+    b = Batch([p], [2])
+    e_auto = b.energies(X).cpu().numpy()
+    sweeps = b.prep_stats()[:, 0]
+    assert np.all(sweeps > 0), 'the Jacobi SVD ran'
+    b.set_arith('complex')
+    e_cx = b.energies(X).cpu().numpy()
+    assert np.all(b.prep_stats()[:, 0] > 0)
+    for i in range(2):
+        sim = mps_ref.RefSimulator(n, max_bond=max_bond, abs_cut=cut, rel_cut=cut).run(tape_with_values(ops, X[i]))
+        trunc, gap, thr_margin = truncation_margins(sim, max_bond, cut)
+        assert (trunc > 0) == truncates and gap > 1e-6 and thr_margin > 1e-2, (trunc, gap, thr_margin)
+        eo, _ = sim.observe(words, coeffs)
+        assert rel_close(e_auto[i], eo, 1e-9), (i, e_auto[i], eo)
+        assert rel_close(e_cx[i], eo, 1e-9), (i, e_cx[i], eo)
+    if set(gates) == {'ry'}:
+        assert p.info()['real_arith'] == 1
+
+
 def test_bond_guard_error(cuda):
     n = 6
     ops, P = workloads.su2_ansatz(n, 3, ('ry',), 'linear')
@@ -177,8 +240,8 @@ def test_spsa_40q_config_parity_and_batch_invariance(cuda):
     single = Batch([p], [1])
     for i in (0, 1, 77, 255):
         assert single.energies(w['x'][i]).cpu().numpy()[0] == e[i]
-    # oracle parity on a few instances
-    for i in (0, 1, 200):
+    # oracle parity on every 16th instance (both members of the first pair)
+    for i in [0, 1] + list(range(16, 256, 16)):
         eo, _ = oracle_terms(w['n'], w['ops'], w['x'][i], w['words'], w['coeffs'])
         assert rel_close(e[i], eo), (i, e[i], eo)
     # SPSA pairs differ (the +/- perturbations are not degenerate)

@@ -5,11 +5,12 @@ Checks, each against something the kernels were not derived from:
     10-78, golden/reference_cases.json) against oracle/reverse_ref and the
     dense statevector: energy within 1e-12, gradient within 1e-8 absolute;
   * random tapes with X/Y/Z words against reverse_ref (1e-8 absolute);
-  * the 60-qubit adjoint workload (8000 Z-strings) against the exact
-    parameter-shift rule evaluated with the independent SVD energy path
-    (E(x + pi/2 e_p) - E(x - pi/2 e_p)) / 2 on 32 random parameters, and
-    against central differences (h = 1e-6) with a tolerance set by the
-    difference's rounding error;
+  * the 60-qubit adjoint workload (8000 Z-strings, batch 32, checkpoint
+    stride 2) against the exact parameter-shift rule evaluated with the
+    independent SVD energy path (E(x + pi/2 e_p) - E(x - pi/2 e_p)) / 2 on
+    32 random parameters, against central differences (h = 1e-6) with a
+    tolerance set by the difference's rounding error, and against the
+    reverse oracle;
   * the value equals the SVD path's energy (1e-10 relative);
   * batched and one-at-a-time gradients are bitwise equal.
 """
@@ -162,41 +163,62 @@ def test_real_path_40q_matches_complex(cuda):
     np.testing.assert_allclose(g.cpu().numpy(), gc.cpu().numpy(), atol=1e-9, rtol=0)
 
 
-def test_adjoint_60q_parameter_shift_and_fd(cuda):
+def test_adjoint_60q_config(cuda):
+    """The 60-qubit adjoint configuration as stated: 4 layers of RY+RZ with
+    linear CX chains (480 parameters), 8000 Z-strings, a batch of 32 vectors
+    with step environments checkpointed every other step (recomputed in the
+    backward pass), checked against
+      * the batch at checkpoint stride 1, bitwise;
+      * exact parameter shift on 32 random parameters of two vectors (1e-8);
+      * central differences, h = 1e-6, on the same 32 parameters;
+      * the reverse oracle (reverse_mps.py:84-99 restated) on one vector at
+        1e-8 absolute, its energy at 1e-10 relative."""
     w = workloads.config(2)
     n, P = w['n'], w['nparams']
     p = Problem(n, w['ops'], P, w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
-    assert p.info()['grad_ok'] == 1
-    x = w['x'][:2]
-    print('case: adjoint-60q value_grad batch 2', flush=True)
-    e, g = Batch([p], [2]).value_grad(x)
-    e, g = e.cpu().numpy(), g.cpu().numpy().reshape(2, P)
+    assert p.info()['grad_ok'] == 1 and P == 480 and len(w['x']) == 32
+    X = w['x']
+    print('case: adjoint-60q value_grad batch 32, checkpoint stride 2 and 1', flush=True)
+    b = Batch([p], [32])
+    b.set_checkpoint(2)
+    e, g = b.value_grad(X)
+    e, g = e.cpu().numpy(), g.cpu().numpy().reshape(32, P)
+    b.set_checkpoint(1)
+    e1, g1 = b.value_grad(X)
+    assert np.array_equal(e1.cpu().numpy(), e) and np.array_equal(g1.cpu().numpy().reshape(32, P), g)
+    del b
     idx = np.random.default_rng(7).choice(P, 32, replace=False)
-    shifts = []
+    h = 1e-6
+    pts = []
+    for row in (0, 1):
+        for sgn in (+1, -1):
+            for k in idx:
+                xs = X[row].copy()
+                xs[k] += sgn * np.pi / 2
+                pts.append(xs)
     for sgn in (+1, -1):
         for k in idx:
-            xs = x[0].copy()
-            xs[k] += sgn * np.pi / 2
-            shifts.append(xs)
-    h = 1e-6
-    fds = []
-    for sgn in (+1, -1):
-        for k in idx[:8]:
-            xs = x[0].copy()
+            xs = X[0].copy()
             xs[k] += sgn * h
-            fds.append(xs)
-    print('case: adjoint-60q SVD-path energies for parameter shift + FD', flush=True)
-    es = Batch([p], [len(shifts) + len(fds) + 2]).energies(np.vstack(shifts + fds + list(x))).cpu().numpy()
-    ps = (es[:32] - es[32:64]) / 2
+            pts.append(xs)
+    print(f'case: adjoint-60q SVD-path energies for parameter shift + FD ({len(pts) + 2} vectors)', flush=True)
+    es = Batch([p], [len(pts) + 2]).energies(np.vstack(pts + [X[0], X[1]])).cpu().numpy()
     # value: the SVD state and the bond-doubling state are the same state
-    assert np.all(np.abs(e - es[-2:]) <= 1e-10 * np.abs(es[-2:]))
-    # parameter shift is exact for single-use rotations (rounding of E only)
-    np.testing.assert_allclose(g[0, idx], ps, atol=1e-8, rtol=0)
-    fd = (es[64:72] - es[72:80]) / (2 * h)
-    # central difference: truncation h^2 |E'''| / 6 is negligible; rounding of
-    # the two energies is bounded by ~10 eps sum|c_k|, divided by 2h
+    assert np.all(np.abs(e[:2] - es[-2:]) <= 1e-10 * np.abs(es[-2:]))
+    for row in (0, 1):
+        base = 64 * row
+        ps = (es[base:base + 32] - es[base + 32:base + 64]) / 2
+        # parameter shift is exact for single-use rotations (rounding of E only)
+        np.testing.assert_allclose(g[row, idx], ps, atol=1e-8, rtol=0)
+    fd = (es[128:160] - es[160:192]) / (2 * h)
+    # central difference: the truncation term h^2 E'''/6 is negligible; the
+    # rounding of the two energies is bounded by ~10 eps sum|c_k|, over 2h
     tol = 10 * np.finfo(float).eps * np.abs(w['coeffs']).sum() / h
-    np.testing.assert_allclose(g[0, idx[:8]], fd, atol=tol, rtol=0)
+    np.testing.assert_allclose(g[0, idx], fd, atol=tol, rtol=0)
+    print('case: adjoint-60q reverse oracle on vector 0 (CPU)', flush=True)
+    eo, go = reverse_ref.value_gradient(n, w['ops'], w['words'], np.real(w['coeffs']), X[0])
+    assert abs(e[0] - eo) <= 1e-10 * abs(eo)
+    np.testing.assert_allclose(g[0], go, atol=1e-8, rtol=0)
 
 
 def test_batched_equals_plain_bitwise(cuda):

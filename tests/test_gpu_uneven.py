@@ -2,8 +2,9 @@
 Z-strings, mixed linear / reverse-chain 4-layer RY ansatz, chi <= 16), energies
 plus 1000 shots each from one state preparation, eagerly, from a captured HIP
 graph, and under every autotuner launch shape.  Every variant must equal the
-sequential per-partition launches bitwise (energies and bitstrings), and a
-sample of partitions is checked against the CPU oracle (1e-10 relative)."""
+sequential per-partition launches bitwise (energies and bitstrings).  Every
+partition's energy is checked against the CPU oracle (1e-10 relative), and
+the bitstrings of the 16 largest partitions against the oracle's draws."""
 import numpy as np
 import pytest
 
@@ -50,13 +51,34 @@ def test_evaluate_matches_sequential_and_oracle(uneven):
     es, bs = sequential(parts, probs, sd)
     assert np.array_equal(e.cpu().numpy(), es)
     same_bits(bits, bs, parts)
-    small = [j for j, q in enumerate(parts) if q['n'] <= 12][:3]
-    for j in small:
-        q = parts[j]
+
+
+def test_evaluate_matches_oracle_every_partition(uneven):
+    """Every partition's energy against the oracle (1e-10 relative), and the
+    1000 bitstrings of the 16 largest partitions (26-32 qubits) against the
+    oracle's draws with the same SplitMix64 stream: identical except where a
+    shot's first difference sits within 1e-12 of a CDF boundary."""
+    parts, probs, x, sd = uneven
+    print('case: uneven-64 evaluate vs oracle', flush=True)
+    batch = Batch(probs, [1] * len(probs))
+    e, bits, _ = batch.evaluate(x, SHOTS, sd)
+    e, bits = e.cpu().numpy(), bits.cpu().numpy()
+    largest = sorted(range(len(parts)), key=lambda j: -parts[j]['n'])[:16]
+    assert min(parts[j]['n'] for j in largest) >= 26
+    differing = 0
+    for j, q in enumerate(parts):
         tape = [(g, qq, (q['x'][0][k] if k is not None else None)) for g, qq, k in q['ops']]
-        ref, _ = mps_ref.RefSimulator(q['n'], max_bond=16, abs_cut=0.0, rel_cut=0.0).run(tape).observe(
-            q['words'], q['coeffs'])
-        assert abs(es[j] - ref) <= 1e-10 * max(1.0, abs(ref))
+        sim = mps_ref.RefSimulator(q['n'], max_bond=16, abs_cut=0.0, rel_cut=0.0).run(tape)
+        ref, _ = sim.observe(q['words'], q['coeffs'])
+        assert abs(e[j] - ref) <= 1e-10 * max(1.0, abs(ref)), (j, q['n'], e[j], ref)
+        if j in largest:
+            ref_bits, margins = sim.mps.sample_bits(SHOTS, sd[j], margins=True)
+            got = bits[j, :, :q['n']]
+            for sh in np.nonzero((got != ref_bits).any(axis=1))[0]:
+                first = int(np.nonzero(got[sh] != ref_bits[sh])[0][0])
+                assert margins[sh, first] < 1e-12, (j, sh, first, margins[sh, first])
+                differing += 1
+    print(f'shots differing from the oracle (all at CDF boundaries): {differing}', flush=True)
 
 
 def test_graph_replay_matches_eager(uneven):
