@@ -211,3 +211,100 @@ def test_backend_metrics_wrapper_composes_with_observe(tmp_path):
     rows = [json.loads(l) for l in log.read_text().splitlines()]
     assert [r['operation'] for r in rows] == ['observe', 'observe_async_until_get']
     assert all(r['parameters'] == x and r['parameter_count'] == count for r in rows)
+
+
+def test_sampled_readout_and_rows_against_oracle(tmp_path):
+    """A sampled campaign run on this backend (sampled_execution.py:10-41 over
+    this module): sample and sample_async through the recorder give the
+    oracle's counts for the same sampleSeed sequence, the protein decoder's
+    probabilities are the reversed keys over 1000 shots (readout.py:4-9),
+    every call leaves one row with its parameters, and state extraction is
+    refused."""
+    import json
+    import types
+    from gvmps import readout
+    n = 8
+    r = np.random.default_rng(21)
+    kernel, ops, count = build(n, 3, ('ry',), 'reverse_linear', spread=True)
+    xs = [r.uniform(-np.pi, np.pi, count).tolist() for _ in range(3)]
+    ns = types.SimpleNamespace(observe=cudaq.observe, observe_async=cudaq.observe_async,
+                               sample=cudaq.sample, sample_async=cudaq.sample_async, get_state=cudaq.get_state)
+    log = tmp_path / 'sampled.jsonl'
+    readout.install_backend_metrics(ns, path=str(log), shots=1000)
+    print('case: sampled readout 8q, 3 sample calls', flush=True)
+    got = [ns.sample(kernel, *xs[0]), ns.sample_async(kernel, *xs[1]).get(), ns.sample(kernel, *xs[2])]
+    with pytest.raises(RuntimeError, match='Dense state extraction is disabled'):
+        ns.get_state(kernel, *xs[0])
+    sim_seed = rng.INITIAL_SAMPLE_SEED
+    for x, counts in zip(xs, got):
+        sim = mps_ref.RefSimulator(n, max_bond=8, abs_cut=0.0, rel_cut=0.0).run(tape_values(ops, x))
+        sim.sample_seed = sim_seed
+        want = sim.sample(1000)
+        sim_seed += 1
+        assert dict(counts) == want
+        probs = readout.protein_probabilities(counts)
+        assert probs == {k[::-1]: v / 1000 for k, v in want.items()}
+        assert abs(sum(probs.values()) - 1.0) < 1e-12
+    rows = [json.loads(l) for l in log.read_text().splitlines()]
+    assert [r_['operation'] for r_ in rows] == ['sample', 'sample_async', 'sample']
+    assert [r_['parameters'] for r_ in rows] == xs
+    assert all(r_['shots'] == 1000 and r_['parameter_count'] == count for r_ in rows)
+
+
+def test_spsa_batched_equals_sequential_observe_loop(cuda):
+    """SPSA (spsa_optimizer.py:66-184): the batched loop, one launch per
+    update, against the sequential restatement making one scalar observe call
+    per probe: identical iterates and result; the best energy equals the
+    oracle's energy of the best point (1e-10 relative)."""
+    from oracle import spsa_ref
+    n = 8
+    r = np.random.default_rng(31)
+    kernel, ops, P = build(n, 3, ('ry',), 'linear')
+    H, words, coeffs = hamiltonian(n, r, 40, xy=False)
+    x0 = r.uniform(-np.pi, np.pi, P)
+    c = optimize.spsa_settings({})
+    c.update(energy_tol=1e-6)
+    print('case: SPSA 8q batched vs sequential observe', flush=True)
+    seq_it, bat_it = [], []
+    want = spsa_ref.minimize(lambda v: cudaq.observe(kernel, H, v).expectation(), x0, c, maxiter=25,
+                             callback=lambda v: seq_it.append(v.copy()))
+    obj = optimize.BatchedObjective(kernel, H)
+    got = optimize.spsa_minimize(obj, x0, maxiter=25, settings=c, callback=lambda v: bat_it.append(v.copy()))
+    assert want['nit'] == 25 and len(seq_it) == len(bat_it) == want['nit']
+    assert all(np.array_equal(a, b) for a, b in zip(seq_it, bat_it))
+    assert np.array_equal(got.x, want['x']) and got.fun == want['fun']
+    assert (got.nit, got.nfev, got.njev, got.message) == (want['nit'], want['nfev'], want['njev'], want['message'])
+    assert obj.launches == got.nit + 1
+    sim = mps_ref.RefSimulator(n, max_bond=8, abs_cut=0.0, rel_cut=0.0).run(tape_values(ops, got.x))
+    e_ref, _ = sim.observe(words, coeffs)
+    assert abs(got.fun - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
+
+
+def test_spsa_many_runs_in_one_launch_per_round(cuda):
+    """Several independent SPSA runs (partitions of different sizes) share
+    each round's launch; every run equals its own batched loop."""
+    from gvmps import Batch, Problem
+
+    class ProblemObjective:
+        def __init__(self, p):
+            self.p = p
+
+        def prefetch(self, pts):
+            pts = np.atleast_2d(pts)
+            return Batch([self.p], [len(pts)]).energies(pts).cpu().numpy()
+
+    r = np.random.default_rng(32)
+    probs, x0s = [], []
+    for n in (6, 9, 12):
+        ops, P = workloads.layered_ansatz(n, 3, ('ry',), 'linear')
+        words, coeffs = workloads.z_hamiltonian(n, 60, 3, r)
+        probs.append(Problem(n, ops, P, words, coeffs, max_bond=16, abs_cut=0, rel_cut=0))
+        x0s.append(r.uniform(-np.pi, np.pi, P))
+    c = optimize.spsa_settings({})
+    c.update(energy_tol=1e-3)
+    print('case: spsa_many over 6/9/12-qubit problems', flush=True)
+    res, rounds = optimize.spsa_many(probs, x0s, maxiter=40, settings=c)
+    assert rounds == max(rr.nit for rr in res) + 1
+    for p, x0, rr in zip(probs, x0s, res):
+        one = optimize.spsa_minimize(ProblemObjective(p), x0, maxiter=40, settings=c)
+        assert np.array_equal(one.x, rr.x) and one.fun == rr.fun and one.nit == rr.nit and one.nfev == rr.nfev
