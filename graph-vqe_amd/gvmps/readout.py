@@ -40,14 +40,22 @@ def _append(path, row):
 def install_backend_metrics(module, path=None, shots=None):
     """Wrap ``module``'s evaluation calls to log one row per evaluation.
 
-    ``shots=None`` is an analytic run: observe calls are forced to
+    ``shots=None`` is an analytic run: observe calls run with
     ``shots_count=-1`` and rows carry ``shots=-1, physical_shots=0`` with the
     engine's bond and cutoff settings.  An integer ``shots`` is a sampled run:
     observe and sample calls are forced to that shot count and rows carry
     ``shots`` and ``physical_shots_per_circuit_or_term``.  The path defaults
     to ``$ANALYTIC_MPS_BACKEND_METRICS`` (else backend_evaluations.jsonl).
     Async calls log when ``get()`` first returns.  Returns the wrapped names.
+
+    The campaign recorders' guards come with them.  An analytic run raises on
+    a finite ``shots_count`` or a noise model and replaces ``sample``,
+    ``sample_async`` and ``get_state`` by functions that raise
+    (analytic_execution.py:63-66, 77-79); a sampled run accepts exactly 1000
+    shots and replaces ``get_state`` (sampled_execution.py:13-14, 40-41).
     """
+    if shots is not None and int(shots) != 1000:
+        raise ValueError('Matched sampled suite requires 1000 shots')
     log_path = path or os.environ.get('ANALYTIC_MPS_BACKEND_METRICS', 'backend_evaluations.jsonl')
 
     def row(operation, start, args):
@@ -81,6 +89,11 @@ def install_backend_metrics(module, path=None, shots=None):
         original = getattr(module, name)
 
         def call(*args, **kwargs):
+            if shots is None:
+                if kwargs.get('shots_count', -1) != -1:
+                    raise RuntimeError('Finite-shot observe forbidden in analytic campaign')
+                if kwargs.get('noise_model') is not None:
+                    raise RuntimeError('Noise model forbidden in analytic campaign')
             kwargs['shots_count'] = -1 if shots is None else int(shots)
             start = time.perf_counter()
             result = original(*args, **kwargs)
@@ -97,4 +110,16 @@ def install_backend_metrics(module, path=None, shots=None):
                    ('sample', 'sample'), ('sample_async', 'sample_async')]
     for name, logged in wrapped:
         wrap(name, logged)
+
+    def forbidden(message):
+        def call(*args, **kwargs):
+            raise RuntimeError(message)
+        return call
+
+    if shots is None:
+        message = 'Sampling/statevector readout forbidden: this campaign records analytic optimization only'
+        for name in ('sample', 'sample_async', 'get_state'):
+            setattr(module, name, forbidden(message))
+    else:
+        setattr(module, 'get_state', forbidden('Dense state extraction is disabled; use sampled readout'))
     return [name for name, _ in wrapped]

@@ -214,7 +214,7 @@ def test_backend_metrics_rows(tmp_path, monkeypatch):
     monkeypatch.setenv('CUDAQ_MPS_MAX_BOND', '16')
     log = tmp_path / 'an.jsonl'
     assert readout.install_backend_metrics(mod, path=str(log)) == ['observe', 'observe_async']
-    assert mod.observe('k', 'H', 0.5, 1.5, shots_count=1000) == 'observe'
+    assert mod.observe('k', 'H', 0.5, 1.5) == 'observe'
     fut = mod.observe_async('k', 'H', [0.25, 0.75], qpu_id=0)
     assert fut.get() == 'async' and fut.get() == 'async'
     rows = [json.loads(l) for l in log.read_text().splitlines()]
@@ -234,3 +234,36 @@ def test_backend_metrics_rows(tmp_path, monkeypatch):
     assert [r['parameters'] for r in rows] == [[0.1, 0.2], [0.3], [0.4]]
     assert all(r['shots'] == 1000 and r['physical_shots_per_circuit_or_term'] == 1000
                and r['added_noise_model'] is False for r in rows)
+
+
+def test_backend_metrics_guards(tmp_path):
+    """The recorders' guards (analytic_execution.py:63-66, 77-79;
+    sampled_execution.py:13-14, 40-41): an analytic run rejects finite shots
+    and noise models and forbids sampling and state extraction; a sampled run
+    needs exactly 1000 shots and forbids state extraction."""
+    import types
+    from gvmps import readout
+
+    def fake(name):
+        return lambda *a, **kw: name
+    names = ('observe', 'observe_async', 'sample', 'sample_async', 'get_state')
+    mod = types.SimpleNamespace(**{n: fake(n) for n in names})
+    readout.install_backend_metrics(mod, path=str(tmp_path / 'a.jsonl'))
+    with pytest.raises(RuntimeError, match='Finite-shot observe forbidden in analytic campaign'):
+        mod.observe('k', 'H', 0.5, shots_count=1000)
+    with pytest.raises(RuntimeError, match='Noise model forbidden in analytic campaign'):
+        mod.observe_async('k', 'H', 0.5, noise_model=object())
+    for name in ('sample', 'sample_async', 'get_state'):
+        with pytest.raises(RuntimeError, match='Sampling/statevector readout forbidden'):
+            getattr(mod, name)('k', 0.5)
+    assert mod.observe('k', 'H', 0.5, shots_count=-1) == 'observe'
+    assert not (tmp_path / 'b.jsonl').exists()
+    for bad in (999, 1, 4096):
+        with pytest.raises(ValueError, match='Matched sampled suite requires 1000 shots'):
+            readout.install_backend_metrics(types.SimpleNamespace(**{n: fake(n) for n in names}),
+                                            path=str(tmp_path / 'b.jsonl'), shots=bad)
+    mod2 = types.SimpleNamespace(**{n: fake(n) for n in names})
+    readout.install_backend_metrics(mod2, path=str(tmp_path / 's.jsonl'), shots=1000)
+    with pytest.raises(RuntimeError, match='Dense state extraction is disabled; use sampled readout'):
+        mod2.get_state('k', 0.5)
+    assert mod2.sample('k', 0.5) == 'sample'
