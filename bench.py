@@ -33,54 +33,107 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(w, budget_s):
-    """Oracle (numpy restatement of the reference algorithm) on this host."""
+def _oracle_energy_worker(args):
+    """Energy evaluations of the 40-qubit workload by the numpy oracle, one
+    BLAS thread, for `budget` seconds (the batch's vectors round robin)."""
+    worker, nworkers, budget = args
     from threadpoolctl import threadpool_limits
+    from gvmps import workloads
     from oracle import mps_ref
-    tape = [(g, q, (w['x'][0][p] if p is not None else None)) for g, q, p in w['ops']]
+    w = workloads.config(1)
     done, t0 = 0, time.perf_counter()
     with threadpool_limits(limits=1):
         while True:
+            x = w['x'][(worker + done * nworkers) % len(w['x'])]
+            tape = [(g, q, (x[p] if p is not None else None)) for g, q, p in w['ops']]
             sim = mps_ref.RefSimulator(w['n'], max_bond=16, abs_cut=0.0, rel_cut=0.0).run(tape)
             sim.observe(w['words'], w['coeffs'])
             done += 1
-            if time.perf_counter() - t0 > budget_s:
+            if time.perf_counter() - t0 > budget:
                 break
-    dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="energy evals/s", cores=1, kind="port",
-                sample=f"{done} full energy evaluation(s) of instance 0 (40 q, 4000 terms) by the numpy "
-                       f"oracle (explicit-SVD MPS + chunk-128 prefix DAG), BLAS limited to 1 thread, {dt:.1f} s")
+    return done, time.perf_counter() - t0
 
 
-def measured_traffic():
-    """HBM bytes per step of the chain kernel from the committed rocprofv3
+def _oracle_grad_worker(args):
+    """Value + gradient evaluations by the torch-CPU reverse oracle, one
+    thread, for `budget` seconds."""
+    worker, nworkers, budget = args
+    import torch
+    from gvmps import workloads
+    from oracle import reverse_ref
+    torch.set_num_threads(1)
+    w = workloads.config(1)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        x = w['x'][(worker + done * nworkers) % len(w['x'])]
+        reverse_ref.value_gradient(w['n'], w['ops'], w['words'], w['coeffs'], x)
+        done += 1
+        if time.perf_counter() - t0 > budget:
+            break
+    return done, time.perf_counter() - t0
+
+
+def cpu_cores():
+    """Host cores for the all-cores baseline: the affinity set, capped by the
+    host's thread budget for one GPU (OMP_NUM_THREADS where it is set)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get('OMP_NUM_THREADS')
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+
+
+def _timed_pool(fn, nworkers, budget_s):
+    """Throughput of `nworkers` single-threaded worker processes (started
+    fresh, before this process touches the GPU) over about budget_s."""
+    import multiprocessing as mp
+    if nworkers == 1:
+        done, dt = fn((0, 1, budget_s))
+        return done, dt
+    ctx = mp.get_context('spawn')
+    t0 = time.perf_counter()
+    with ctx.Pool(nworkers) as pool:
+        res = pool.map(fn, [(i, nworkers, budget_s) for i in range(nworkers)])
+    wall = time.perf_counter() - t0
+    # each worker's own clock excludes its interpreter start-up
+    rate = sum(d / t for d, t in res)
+    return sum(d for d, _ in res), sum(d for d, _ in res) / rate if rate > 0 else wall
+
+
+def cpu_baselines(budget_s, with_grad):
+    """The CPU path timed on this host: the numpy oracle (explicit-SVD MPS +
+    chunk-128 prefix DAG, the reference's algorithm) and the torch-CPU
+    reverse oracle, on one core and on every core of the host (one
+    single-threaded process per core, independent evaluations of the batch)."""
+    out = {}
+    ncores = cpu_cores()
+    for key, fn, unit, what in (
+            ('energy', _oracle_energy_worker, 'energy evals/s',
+             'energy evaluation(s) of the 40 q, 4000-term workload by the numpy oracle (explicit-SVD MPS + '
+             'chunk-128 prefix DAG)'),
+            ('grad', _oracle_grad_worker, 'adjoint grads/s',
+             'value+gradient evaluation(s) (40 q, 160 params, 4000 terms) by the torch-CPU restatement of the '
+             'reverse engine (bond doubling, chunk 128)')):
+        if key == 'grad' and not with_grad:
+            continue
+        d1, t1 = _timed_pool(fn, 1, budget_s)
+        dn, tn = _timed_pool(fn, ncores, budget_s)
+        out[key] = dict(value=dn / tn, unit=unit, cores=ncores, kind="port",
+                        sample=f"{dn} {what}, {ncores} single-threaded processes, {budget_s:.0f} s each",
+                        single_core=dict(value=d1 / t1, cores=1, sample=f"{d1} {what}, 1 thread, {t1:.1f} s"))
+    return out
+
+
+def measured_traffic(key='hbm_bytes_per_launch:'):
+    """HBM bytes per launch of the chain kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
     path = os.path.join(ROOT, 'profiles', 'rchains_energy_spsa40q_pmc.txt')
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith('hbm_bytes_per_step:'):
+                if line.startswith(key):
                     return float(line.split(':')[1])
     except OSError:
         pass
     return None
-
-
-def cpu_grad_baseline(w, budget_s):
-    """Torch-CPU restatement of the reference's reverse engine on this host."""
-    import torch
-    from oracle import reverse_ref
-    torch.set_num_threads(1)
-    done, t0 = 0, time.perf_counter()
-    while True:
-        reverse_ref.value_gradient(w['n'], w['ops'], w['words'], w['coeffs'], w['x'][done % len(w['x'])])
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="adjoint grads/s", cores=1, kind="port",
-                sample=f"{done} value+gradient evaluation(s) (40 q, 160 params, 4000 terms) by the torch-CPU "
-                       f"restatement of the reverse engine (bond doubling, chunk 128), 1 thread, {dt:.1f} s")
 
 
 def main():
@@ -104,9 +157,13 @@ def main():
                               requested_gpus=args.gpus)))
         sys.exit(2)
 
+    # the CPU path first, in fresh single-threaded processes, before this
+    # process touches the GPU
+    cpu = None if args.no_cpu else cpu_baselines(args.cpu_budget, not args.no_grad)
+
     import numpy as np
     import torch
-    from gvmps import Batch, Problem, workloads
+    from gvmps import Batch, Problem, workcount, workloads
 
     torch.cuda.set_device(0)
     w = workloads.config(1)
@@ -131,17 +188,34 @@ def main():
     plain_rate, same = plain_rate_of(lambda b, v: b.energies(v), Batch([prob], [1]), x[: args.plain_sample],
                                      lambda i, r: torch.equal(r, e[i:i + 1]))
 
-    # dominant kernel: forest chains (all generations of one step)
+    # dominant kernel: the forest-chain kernel (its 6 generation launches per
+    # step), timed per launch by HIP events around each launch on the launch
+    # stream (slots 6, 7); the chains phase adds the site packing and the
+    # leaf term-end launch
     chains_ms = phase_ms[2] / max(calls, 1)
-    # algorithmic flops: 16 (Dl^2 Dr + Dl Dr^2) per forest node in complex
-    # arithmetic (prefix and suffix forests, the planner's node count); the
-    # circuit is real (RY + CX, Z-strings), so the forests run in real
-    # arithmetic and the count is 4 (Dl^2 Dr + Dl Dr^2): a quarter
+    kern_ms, kern_launches = phase_ms[6], int(phase_ms[7])
+    kern_ms_step = kern_ms / max(calls, 1)
+    # executed MFMA flops of the chain kernel: the planner counts (4 or 6)
+    # chi^3 complex multiply-adds per expansion step at 8 flops each; the real
+    # path does one real product per complex product (a quarter of that), the
+    # complex path three real products (Gauss form, three quarters)
     real = bool(info['real_arith'])
-    arith_div = 4.0 if real else 1.0
-    node_flops = info['flops_energy_nodes'] / arith_div
-    achieved = (node_flops * B) / (chains_ms * 1e-3) / 1e12
-    executed = (info['flops_energy_executed'] / arith_div * B) / (chains_ms * 1e-3) / 1e12
+    exec_per_eval = info['flops_energy_executed'] * (0.25 if real else 0.75)
+    achieved = exec_per_eval * B / (kern_ms_step * 1e-3) / 1e12
+    # the algorithm's own work: left/right environments with shared prefixes,
+    # counted from the workload (gvmps.workcount); the kernels do less
+    # arithmetic than that (real state, prefix and suffix forests with
+    # sibling sharing): reported as a reduction, not as a roofline fraction
+    wc = workcount.energy_work(w['n'], w['ops'], w['words'])
+    algorithmic = {
+        "flops_per_eval": wc['flops_per_eval'], "node_site_transfers": wc['node_site_transfers'],
+        "reference_chunk128_dag_flops_per_eval": wc['reference_dag_flops_per_eval'],
+        "executed_flops_per_eval": exec_per_eval,
+        "reduction_vs_algorithmic": round(wc['flops_per_eval'] / exec_per_eval, 2),
+        "basis": "16 (Dl^2 Dr + Dl Dr^2) flops per environment transfer, N_s distinct prefixes per site over "
+                 "the words' spans, plus the left and right identity environments",
+    }
+    traffic = measured_traffic()
 
     # the same workload on the complex kernels (the path of complex circuits)
     complex_leg = None
@@ -158,9 +232,12 @@ def main():
         cx_rate = B * args.steps / (time.perf_counter() - t1)
         cx_ms, cx_calls = batch.timing()
         cx_chains = cx_ms[2] / max(cx_calls, 1)
-        cx_flops = info['flops_energy_nodes'] * B / (cx_chains * 1e-3) / 1e12
+        cx_kern = cx_ms[6] / max(cx_calls, 1)
+        cx_flops = info['flops_energy_executed'] * 0.75 * B / (cx_kern * 1e-3) / 1e12
         complex_leg = {"evals_per_s": round(cx_rate, 2), "forest_chains_ms": round(cx_chains, 3),
+                       "chain_kernel_ms": round(cx_kern, 3),
                        "achieved_tflops": round(cx_flops, 3), "frac": round(cx_flops / PEAK_FP64_TFLOPS, 4),
+                       "flops_basis": "executed MFMA flops (3 real products per complex product)",
                        "max_rel_diff_vs_real": float(torch.max(torch.abs(e_cx - e) / torch.clamp(torch.abs(e), min=1.0))),
                        "kernel": ("k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy")
                                  + " + k_leaf_energy (complex128, Gauss-form MFMA products)"}
@@ -204,17 +281,22 @@ def main():
                           "note": "same energies, state built by exact CNOT bond doubling instead of SVDs"},
         "phase_ms_per_step": {"state_prep": round(phase_ms[0] / max(calls, 1), 3),
                               "forest_chains": round(chains_ms, 3),
+                              "chain_kernel": round(kern_ms_step, 3),
                               "reduce": round(phase_ms[3] / max(calls, 1), 3)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "traffic": measured_traffic(),
-                     "traffic_basis": "HBM bytes per step (all generations), rocprofv3 PMC, copy-calibrated",
-                     "kernel": (("k_rchains_energy_iz (real arithmetic, I/Z-only forest)" if info['iz_forest']
-                                 else "k_rchains_energy (real arithmetic)") if real
-                                else "k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy")
-                               + " (all generations of one step) + leaf term-end kernel",
-                     "algorithmic_flops_per_step": node_flops * B,
-                     "executed_tflops": round(executed, 3)},
+                     "traffic": traffic,
+                     "traffic_basis": "HBM bytes per launch of the chain kernel (mean of its generation launches), "
+                                      "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated",
+                     "kernel": ("k_rchains_energy_iz" if info['iz_forest'] else "k_rchains_energy") if real
+                               else ("k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy"),
+                     "launches_per_step": kern_launches // max(calls, 1),
+                     "avg_launch_us": round(kern_ms / max(kern_launches, 1) * 1e3, 1),
+                     "flops_basis": "executed MFMA flops of the kernel per step: " + (
+                         "one real 16x16x16 product per complex product (real state)" if real else
+                         "three real products per complex product (Gauss form)"),
+                     "executed_flops_per_step": exec_per_eval * B},
+        "algorithmic": algorithmic,
         "complex_path": complex_leg,
     }
     if grad is not None:
@@ -222,10 +304,10 @@ def main():
         out["adjoint_60q"] = bench_adjoint60(args)
     if not args.no_uneven:
         out["uneven_batch"] = bench_uneven(args)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
-        if grad is not None:
-            out["adjoint"]["cpu_baseline"] = cpu_grad_baseline(w, args.cpu_budget)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu['energy']
+        if grad is not None and 'grad' in cpu:
+            out["adjoint"]["cpu_baseline"] = cpu['grad']
     print(json.dumps(out), flush=True)
 
 
@@ -369,6 +451,7 @@ def bench_grads(args, prob, info, x, B):
         "checkpoint_stride": args.grad_checkpoint,
         "chains_tflops": round(achieved, 3),
         "chains_frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 4),
+        "backward_kernel_ms_per_step": round(phase_ms[6] / c, 3),
     }
 
 

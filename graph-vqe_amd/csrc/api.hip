@@ -50,6 +50,8 @@ Batch::~Batch() {
   if (cap_stream) hipStreamDestroy(cap_stream);
   for (hipEvent_t &e : ev)
     if (e) hipEventDestroy(e);
+  for (hipEvent_t &e : kev)
+    if (e) hipEventDestroy(e);
   for (void *p : owned) hipFree(p);
   if (sbits) hipFree(sbits);
   if (spacked) hipFree(spacked);
@@ -64,8 +66,31 @@ void Batch::mark(int i, hipStream_t s) {
   ev_used = std::max(ev_used, i + 1);
 }
 
+void Batch::kbegin(hipStream_t s) {
+  if (!timing) return;
+  if ((int)kev.size() < kev_used + 2) kev.resize(kev_used + 2, nullptr);
+  if (!kev[kev_used]) hip_check(hipEventCreate(&kev[kev_used]), "hipEventCreate");
+  hip_check(hipEventRecord(kev[kev_used], s), "hipEventRecord");
+}
+
+void Batch::kend(hipStream_t s) {
+  if (!timing) return;
+  if (!kev[kev_used + 1]) hip_check(hipEventCreate(&kev[kev_used + 1]), "hipEventCreate");
+  hip_check(hipEventRecord(kev[kev_used + 1], s), "hipEventRecord");
+  kev_used += 2;
+}
+
 void Batch::collect() {
-  if (!timing || ev_used < 2) return;
+  if (!timing) return;
+  for (int i = 0; i + 1 < kev_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, kev[i], kev[i + 1]) == hipSuccess) {
+      kern_ms += ms;
+      ++kern_launches;
+    }
+  }
+  kev_used = 0;
+  if (ev_used < 2) return;
   for (int i = 0; i + 1 < ev_used; ++i) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) phase_ms[i] += ms;
@@ -350,10 +375,12 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
+      B.kbegin(stream);
       hip_check(kl_rchains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total, B.leafs,
                            B.spills,
                            terms ? B.terms : nullptr, B.parts),
                 "k_rchains launch");
+      B.kend(stream);
     }
     hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills,
                              terms ? B.terms : nullptr, B.parts),
@@ -368,6 +395,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
   for (int g = 0; g < B.max_gen; ++g) {
     const int ni = B.n_items[g];
     if (ni == 0) continue;
+    B.kbegin(stream);
     if (terms)
       hip_check(kl_chains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.arena, B.n_site_total, B.leafs,
                           B.spills, B.terms),
@@ -377,6 +405,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
                                  B.leafs, B.spills,
                                  B.parts),
                 "k_chains_energy launch");
+    B.kend(stream);
   }
   hip_check(kl_leaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills,
                           terms ? B.terms : nullptr, B.parts),
@@ -511,10 +540,12 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     for (int g = B.max_ggen - 1; g >= 0; --g) {
       const int ni = B.gn_items[g];
       if (ni == 0) continue;
+      B.kbegin(stream);
       hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                B.spills, B.adjspill,
                                B.estore, B.gleafs, B.aslot),
                 "k_rchains_bwd launch");
+      B.kend(stream);
     }
     B.mark(4, stream);
     hip_check(kl_rsite_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_rsite_reduce launch");
@@ -542,10 +573,12 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   for (int g = B.max_ggen - 1; g >= 0; --g) {
     const int ni = B.gn_items[g];
     if (ni == 0) continue;
+    B.kbegin(stream);
     hip_check(kl_chains_bwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
                             B.adjspill, B.estore,
                             B.gleafs, B.aslot),
               "k_chains_bwd launch");
+    B.kend(stream);
   }
   B.mark(4, stream);
   hip_check(kl_site_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_site_reduce launch");
@@ -736,7 +769,10 @@ int gvmps_batch_set_timing(gvmps_batch *b, int enable) {
     Batch &B = *reinterpret_cast<Batch *>(b);
     B.timing = enable != 0;
     B.ev_used = 0;
+    B.kev_used = 0;
     B.phase_calls = 0;
+    B.kern_ms = 0.0;
+    B.kern_launches = 0;
     for (double &v : B.phase_ms) v = 0.0;
   })
 }
@@ -744,7 +780,9 @@ int gvmps_batch_set_timing(gvmps_batch *b, int enable) {
 int gvmps_batch_get_timing(gvmps_batch *b, double *phase_ms, long long *calls) {
   GV_TRY({
     Batch &B = *reinterpret_cast<Batch *>(b);
-    for (int i = 0; i < 8; ++i) phase_ms[i] = B.phase_ms[i];
+    for (int i = 0; i < 6; ++i) phase_ms[i] = B.phase_ms[i];
+    phase_ms[6] = B.kern_ms;
+    phase_ms[7] = (double)B.kern_launches;
     if (calls) *calls = B.phase_calls;
   })
 }

@@ -86,6 +86,15 @@ struct Batch {
   long long phase_calls = 0;
   void mark(int i, hipStream_t s);
   void collect();
+  // per-launch timing of the dominant kernel (the forest-chain kernel of
+  // energy calls, the backward chain kernel of value_grad): events recorded
+  // before and after each of its launches on the launch stream
+  std::vector<hipEvent_t> kev;
+  int kev_used = 0;
+  double kern_ms = 0.0;
+  long long kern_launches = 0;
+  void kbegin(hipStream_t s);
+  void kend(hipStream_t s);
   ~Batch();
 };
 

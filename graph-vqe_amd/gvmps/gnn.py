@@ -174,7 +174,7 @@ class WarmStartGNN:
     @classmethod
     def pretrained(cls, device=None):
         """The network trained by `python -m gvmps.learned_warmstart` (2000
-        Hamiltonians of 20-40 qubits; evaluation in data/learned_warmstart_eval.json)."""
+        Hamiltonians of 20-40 qubits)."""
         import os
         net = cls(device=device)
         net.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data', 'warmstart_gnn.npz'))

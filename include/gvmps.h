@@ -195,8 +195,12 @@ int gvmps_batch_replay(gvmps_batch *b, void *stream);
 /* Per-phase device timing with HIP events recorded on the launch stream.
  * Energy calls: 0 state preparation, 2 forest chains, 3 energy reduction.
  * value_grad: 0 bond-doubling build, 1 forward chains, 2 energy reduction,
- * 3 backward chains, 4 site-adjoint reduction, 5 local backward.  enable != 0 resets the accumulators.  No reference
- * counterpart (the reference times whole calls from Python,
+ * 3 backward chains, 4 site-adjoint reduction, 5 local backward.
+ * Slots 6 and 7: the summed device time (ms) and the number of launches of
+ * the dominant kernel alone, from events recorded right before and after
+ * each of its launches (energy calls: the forest-chain kernel; value_grad:
+ * the backward chain kernel).  enable != 0 resets the accumulators.  No
+ * reference counterpart (the reference times whole calls from Python,
  * test/tools/analytic_execution.py:53-62); used by the benchmark. */
 int gvmps_batch_set_timing(gvmps_batch *b, int enable);
 int gvmps_batch_get_timing(gvmps_batch *b, double *phase_ms /* 8 */, long long *calls);
