@@ -74,11 +74,15 @@ def _oracle_grad_worker(args):
 
 
 def cpu_cores():
-    """Host cores for the all-cores baseline: the affinity set, capped by the
-    host's thread budget for one GPU (OMP_NUM_THREADS where it is set)."""
+    """Worker processes of the all-cores baseline: the affinity set, capped by
+    the host's thread budget for one GPU (OMP_NUM_THREADS where it is set),
+    less this process, which stays alive beside the workers (a process that
+    imports torch counts as a user of the GPU, and one GPU admits 16)."""
     n = len(os.sched_getaffinity(0))
     cap = os.environ.get('OMP_NUM_THREADS')
-    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+    if cap and cap.isdigit():
+        n = min(n, int(cap))
+    return max(1, min(n, 16) - 1) if n > 1 else 1
 
 
 def _timed_pool(fn, nworkers, budget_s):
