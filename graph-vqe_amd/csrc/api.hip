@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,6 +22,35 @@ using namespace gv;
 namespace {
 thread_local std::string g_error;
 }
+
+// Scratch of gvmps_batch_shot_energy_noalloc: the packed shot words (two
+// 64-bit words per shot per instance) between the sampling and the Z-sum
+// kernels.  A file-scope device array, so that entry point needs no
+// allocation: every GPU the library runs on holds its own 32 MiB copy until
+// the process ends.  Host code must not pass the array's name to a kernel
+// (on the host it is only a stand-in for the symbol); it uses the device
+// address that hipGetSymbolAddress returns, looked up once per device.
+#define GV_SCRATCH_WORDS (4u << 20)
+#define GV_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned long long scratch[GV_SCRATCH_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_ptr[GV_SCRATCH_DEVICES] = {};
+
+// Device address of this device's copy of `scratch`; `device` must be current.
+unsigned long long *scratch_address(int device) {
+  if (device < 0 || device >= GV_SCRATCH_DEVICES) throw std::runtime_error("device index out of range for scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    gv::hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[device] = static_cast<unsigned long long *>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 namespace gv {
 
@@ -249,6 +279,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->spills = upload<double2>(B->owned, nullptr, spill);
   B->terms = upload<double2>(B->owned, nullptr, term);
   B->status = upload<int>(B->owned, nullptr, B->inst.size());
+  B->status_host.assign(B->inst.size(), 0);
   B->bonds = upload<int>(B->owned, nullptr, B->inst.size() * (GV_NMAX + 1));
   B->stats = upload<int>(B->owned, nullptr, B->inst.size() * GV_STATS);
   B->energies = upload<double>(B->owned, nullptr, B->inst.size());
@@ -334,7 +365,7 @@ void Batch::set_checkpoint(int stride) {
 }
 
 void check_status(Batch &B, hipStream_t stream) {
-  std::vector<int> st(B.inst.size());
+  std::vector<int> &st = B.status_host;
   hip_check(hipMemcpyAsync(st.data(), B.status, sizeof(int) * st.size(), hipMemcpyDeviceToHost, stream),
             "hipMemcpyAsync status");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -750,6 +781,44 @@ int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots,
     hip_check(hipSetDevice(B.device), "hipSetDevice");
     unsigned long long *packed = launch_sample(B, params_dev, shots, seeds, nullptr, nullptr, s);
     const int ninst = (int)B.inst.size();
+    hip_check(kl_zsums(ninst, B.max_terms, shots, s, B.d_inst, B.d_probs, B.d_masks, packed, B.terms), "k_zsums");
+    hip_check(kl_reduce(ninst, s, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
+    B.mark(4, s);
+    if (energies_dev)
+      hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * B.inst.size(),
+                               hipMemcpyDeviceToDevice, s),
+                "copy energies");
+    if (terms_dev)
+      hip_check(hipMemcpyAsync(terms_dev, B.terms, sizeof(double2) * B.n_terms_total, hipMemcpyDeviceToDevice, s),
+                "copy terms");
+    check_status(B, s);
+  })
+}
+
+int gvmps_batch_shot_energy_noalloc(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
+                                    double *energies_dev, double *terms_dev, void *stream) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    hipStream_t s = (hipStream_t)stream;
+    for (int z : B.zonly)
+      if (!z) throw std::runtime_error("finite-shot observe supports I/Z Pauli words only");
+    if (shots <= 0) throw std::runtime_error("shared bitstring sampling requires a positive shot count");
+    if (!seeds) throw std::runtime_error("null seed array");
+    const int ninst = (int)B.inst.size();
+    if ((unsigned long long)ninst * (unsigned long long)shots * 2ULL > GV_SCRATCH_WORDS)
+      throw std::runtime_error("gvmps_batch_shot_energy_noalloc: instances x shots exceeds the scratch maximum of " +
+                               std::to_string(GV_SCRATCH_WORDS / 2) + " shot words (use gvmps_batch_shot_energy)");
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    unsigned long long *packed = scratch_address(B.device);
+    hip_check(hipMemcpyAsync(B.seeds, seeds, sizeof(unsigned) * ninst, hipMemcpyHostToDevice, s), "seeds H2D");
+    launch_prepare(B, params_dev, s);
+    hip_check(kl_envs(ninst, s, B.d_inst, B.arena, B.envs), "k_envs launch");
+    B.mark(2, s);
+    // every live shot's two words are written before k_zsums reads them, so
+    // nothing in the scratch needs clearing
+    hip_check(kl_sample(ninst, shots, B.max_n, s, B.d_inst, B.arena, B.envs, B.seeds, nullptr, packed),
+              "k_sample launch");
+    B.mark(3, s);
     hip_check(kl_zsums(ninst, B.max_terms, shots, s, B.d_inst, B.d_probs, B.d_masks, packed, B.terms), "k_zsums");
     hip_check(kl_reduce(ninst, s, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
     B.mark(4, s);

@@ -25,6 +25,7 @@ struct Batch {
   double2 *arena = nullptr, *envs = nullptr, *spills = nullptr, *terms = nullptr;
   double *parts = nullptr;  // chain partial energies
   int *status = nullptr, *bonds = nullptr, *stats = nullptr;
+  std::vector<int> status_host;  // read-back of `status`, sized once at creation
   double *energies = nullptr;
   // sampling
   const unsigned long long **d_masks = nullptr;  // per problem: 2 words per term (site order)

@@ -10,6 +10,8 @@
 //             (seed, site, shot); V <- W_bit / sqrt(p_bit).  This restates
 //             k_sampleW / k_sampleProbabilities / k_chooseSample
 //             (sharded_mps.cuh:103-188) with all shots of a block in one wave.
+//             `bits` (one byte per site) may be null: then only the packed
+//             words are written.
 //   k_zsums   one thread per (instance, term): sum over shots of
 //             (-1)^popcount(bits & mask) (k_zSampleSums, sharded_mps.cuh:193-214),
 //             on bit-packed shot words instead of one byte per site.
@@ -100,7 +102,7 @@ k_sample(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict
       vi[t] = (sel ? wi[1][t] : wi[0][t]) * inv;
     }
     if (live && g == 0) {
-      bits[((size_t)ii * shots + shot) * stride + k] = (unsigned char)sel;
+      if (bits) bits[((size_t)ii * shots + shot) * stride + k] = (unsigned char)sel;
       if (sel) {
         if (k < 64) word0 |= 1ULL << k;
         else word1 |= 1ULL << (k - 64);
@@ -111,7 +113,8 @@ k_sample(const InstDev *__restrict__ insts, int ninst, const double2 *__restrict
     packed[((size_t)ii * shots + shot) * 2 + 0] = word0;
     packed[((size_t)ii * shots + shot) * 2 + 1] = word1;
     // rows of smaller instances in a mixed batch: zero the padding columns
-    for (int k = n; k < stride; ++k) bits[((size_t)ii * shots + shot) * stride + k] = 0;
+    if (bits)
+      for (int k = n; k < stride; ++k) bits[((size_t)ii * shots + shot) * stride + k] = 0;
   }
 }
 

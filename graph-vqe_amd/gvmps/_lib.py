@@ -50,6 +50,9 @@ _PROTOS = {
                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_shot_energy': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    'gvmps_batch_shot_energy_noalloc': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p],
     'gvmps_batch_set_timing': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_get_timing': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
     'gvmps_batch_prep_stats': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],

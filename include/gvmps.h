@@ -122,6 +122,18 @@ int gvmps_batch_sample(gvmps_batch *b, const double *params_dev, int shots, cons
 int gvmps_batch_shot_energy(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
                             double *energies_dev, double *terms_dev, void *stream);
 
+/* gvmps_batch_shot_energy without allocation: the same kernels and the same
+ * results (bitwise), with the shots' packed words staged in a file-scope
+ * device array (32 MiB per GPU, held until the process ends) instead of a
+ * buffer the batch allocates on first use; the call makes no hipMalloc,
+ * hipFree or other allocating call.  instances x shots may not exceed
+ * 2,097,152 (error otherwise).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). */
+int gvmps_batch_shot_energy_noalloc(gvmps_batch *b, const double *params_dev, int shots, const uint32_t *seeds,
+                                    double *energies_dev, double *terms_dev, void *stream);
+
 /* Energy and gradient dE/dx of every instance by one reverse (adjoint) pass.
  * Replaces make_problem(kernel, H).value_gradient(x) of the reference's
  * reverse engine (test/tools/reverse_mps.py:84-121; tape variant :141-184):
