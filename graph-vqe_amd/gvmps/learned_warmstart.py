@@ -1,6 +1,7 @@
 """Learned warm start for the global L-BFGS-B solve.
 
     python -m gvmps.learned_warmstart --train 2000 --eval 100 --out ws.json
+    python -m gvmps.learned_warmstart --evaluate-pretrained --eval 100 --seed 4049 --out eval.json
 
 Pipeline:
   1. Training data are (Hamiltonian, converged parameters) pairs. The plain
@@ -16,7 +17,10 @@ Pipeline:
   3. Evaluation uses held-out 40-qubit Hamiltonians. For each start, the
      global L-BFGS-B iterations and the final energy are recorded, from the
      network's angles and from the partition warm start. The MPS energy
-     (complex128) judges both.
+     judges both.  compare_starts counts, per Hamiltonian, the learned
+     route's iterations until it reaches the partition route's final
+     energy (charging a run that never does its own iterations plus the
+     partition route's), under the campaign's L-BFGS-B rule.
 All optimizer runs of one stage are batched by optimize.lbfgs_many.
 """
 import argparse
@@ -31,6 +35,11 @@ from .engine import Problem
 
 REPS, GATES, PATTERN = 3, ('ry',), 'linear'
 LBFGS = dict(maxiter=300, ftol=1e-9, gtol=1e-3)
+# The campaign's L-BFGS-B adapter (test/tools/lbfgs_optimizer.py:58-67,
+# 172-186 with run_case.py's energy_tolerance 1): ftol 1e-9, gtol 1e-3, no
+# iteration cap, and the immediate energy stop as soon as two consecutive
+# objective evaluations differ by less than 1; local and global solves alike.
+CAMPAIGN_LBFGS = dict(maxiter=100000, ftol=1e-9, gtol=1e-3, immediate_tol=1.0)
 
 
 def log(msg):
@@ -57,10 +66,13 @@ def problem(n, words, coeffs):
     return Problem(n, ops, P, words, coeffs, max_bond=16, abs_cut=0.0, rel_cut=0.0)
 
 
-def partition_starts(hams, seed, chunk=256, local_maxiter=100, theta0s=None):
+def partition_starts(hams, seed, chunk=256, local_maxiter=100, theta0s=None, rule=None):
     """Partition warm starts of every Hamiltonian; local solves batched.  The
     local solves start from theta0s[h] (default: the reference's seeded
-    uniform draw, independent_vqe.py:261-263)."""
+    uniform draw, independent_vqe.py:261-263); `rule` replaces their
+    L-BFGS-B options (default ftol 1e-12, gtol 1e-8, local_maxiter, as in
+    independent_vqe.py:233-241)."""
+    opts = dict(maxiter=local_maxiter, ftol=1e-12, gtol=1e-8) if rule is None else dict(rule)
     starts, local_problems, local_x0, owners = [], [], [], []
     for h, (n, words, coeffs) in enumerate(hams):
         P = (REPS + 1) * n * len(GATES)
@@ -78,24 +90,74 @@ def partition_starts(hams, seed, chunk=256, local_maxiter=100, theta0s=None):
         starts.append(theta)
     local_nit = np.zeros(len(hams), dtype=int)
     for a in range(0, len(local_problems), chunk):
-        res, _ = optimize.lbfgs_many(local_problems[a:a + chunk], local_x0[a:a + chunk], maxiter=local_maxiter,
-                                     ftol=1e-12, gtol=1e-8)
+        res, _ = optimize.lbfgs_many(local_problems[a:a + chunk], local_x0[a:a + chunk], **opts)
         for r, (h, idx) in zip(res, owners[a:a + chunk]):
             starts[h][idx] = r.x
             local_nit[h] += r.nit
     return starts, local_nit
 
 
-def solve(hams, starts, chunk=128, label=None):
-    """Global L-BFGS-B from `starts`: [(x, energy, nit, nfev)]."""
+def solve(hams, starts, chunk=128, label=None, rule=None, full=False):
+    """Global L-BFGS-B from `starts`: [(x, energy, nit, nfev)] (full: the
+    scipy results, with their per-iterate energies)."""
     out = []
     for a in range(0, len(hams), chunk):
         probs = [problem(*h) for h in hams[a:a + chunk]]
-        res, rounds = optimize.lbfgs_many(probs, starts[a:a + chunk], **LBFGS)
-        out.extend((r.x, float(r.fun), int(r.nit), int(r.nfev)) for r in res)
+        res, rounds = optimize.lbfgs_many(probs, starts[a:a + chunk], **(LBFGS if rule is None else rule))
+        out.extend(res if full else [(r.x, float(r.fun), int(r.nit), int(r.nfev)) for r in res])
         if label:
             log(f'{label}: {len(out)}/{len(hams)} solved ({rounds} batched rounds)')
     return out
+
+
+def iterations_to_reach(iterate_energies, target, tol):
+    """First accepted iteration whose energy is within tol of target or below
+    (0 = the start), or None."""
+    for k, e in enumerate(iterate_energies):
+        if e <= target + tol:
+            return k
+    return None
+
+
+def compare_starts(net, hams, seed, rule=CAMPAIGN_LBFGS, reach_rtol=1e-6):
+    """Learned start against the partition warm start under one L-BFGS-B
+    rule, at equal final energy.
+
+    For every Hamiltonian both routes run the global L-BFGS-B (the partition
+    route after its local solves).  E_p is the partition route's final
+    energy and N_p its global iterations.  The learned route's cost is the
+    number of its iterations until an iterate is within reach_rtol * sum|c|
+    of E_p or below; when it never gets there, it is charged its own
+    iterations plus N_p (it ends in a worse basin and the partition route
+    has to be run as well).  cut = 1 - mean(charged) / mean(N_p)."""
+    part_starts, local_nit = partition_starts(hams, seed, rule=rule)
+    part = solve(hams, part_starts, rule=rule, full=True)
+    learned = solve(hams, predict_all(net, hams), rule=rule, full=True)
+    rows = []
+    for h, (p, l) in enumerate(zip(part, learned)):
+        scale = float(np.sum(np.abs(hams[h][2])))
+        reach = iterations_to_reach(l.iterate_energies, float(p.fun), reach_rtol * scale)
+        rows.append(dict(n=hams[h][0], terms=len(hams[h][1]), e_partition=float(p.fun), nit_partition=int(p.nit),
+                         local_nit_partition=int(local_nit[h]), e_learned=float(l.fun), nit_learned=int(l.nit),
+                         learned_reach=reach, charged_learned=reach if reach is not None else int(l.nit) + int(p.nit),
+                         e_learned_start=float(l.iterate_energies[0]), scale=scale))
+    nit_p = np.array([r['nit_partition'] for r in rows], dtype=float)
+    charged = np.array([r['charged_learned'] for r in rows], dtype=float)
+    reached = [r for r in rows if r['learned_reach'] is not None]
+    e_p = np.array([r['e_partition'] for r in rows])
+    e_l = np.array([r['e_learned'] for r in rows])
+    scale = np.array([r['scale'] for r in rows])
+    summary = dict(
+        hamiltonians=len(rows), rule={k: v for k, v in rule.items()},
+        mean_nit_partition=float(nit_p.mean()), mean_local_nit_partition=float(np.mean(local_nit)),
+        mean_charged_learned=float(charged.mean()), iteration_cut_at_equal_energy=float(1 - charged.mean() / nit_p.mean()),
+        reached_fraction=len(reached) / len(rows),
+        mean_reach_when_reached=float(np.mean([r['learned_reach'] for r in reached])) if reached else None,
+        mean_nit_learned=float(np.mean([r['nit_learned'] for r in rows])),
+        mean_e_partition=float(e_p.mean()), mean_e_learned=float(e_l.mean()),
+        mean_rel_energy_diff=float(np.mean((e_l - e_p) / scale)),
+        learned_not_worse_fraction=float(np.mean(e_l <= e_p + reach_rtol * scale)))
+    return summary, rows
 
 
 def magnetization_targets(hams, thetas):
@@ -230,8 +292,24 @@ def main(argv=None):
     ap.add_argument('--seed', type=int, default=2026)
     ap.add_argument('--out', default='learned_warmstart.json')
     ap.add_argument('--weights', default='warmstart_gnn.npz')
+    ap.add_argument('--evaluate-pretrained', action='store_true',
+                    help='only compare the shipped network with the partition start on --eval held-out '
+                         'Hamiltonians of --eval-n qubits (seed --seed), under the campaign rule and without '
+                         'the immediate energy stop')
     args = ap.parse_args(argv)
     t0 = time.time()
+    if args.evaluate_pretrained:
+        eval_h = hamiltonians(args.eval, args.eval_n, args.eval_n, args.tmin, args.tmax, args.seed)
+        net = G.WarmStartGNN.pretrained()
+        report = dict(args=vars(args))
+        for name, rule in (('campaign_rule', CAMPAIGN_LBFGS), ('no_immediate_stop', LBFGS)):
+            summary, rows = compare_starts(net, eval_h, args.seed + 7, rule=rule)
+            log(f'{name}: {json.dumps(summary)}')
+            report[name] = dict(summary=summary, rows=rows)
+        report['seconds'] = time.time() - t0
+        with open(args.out, 'w') as f:
+            json.dump(report, f)
+        return report
     train_h = hamiltonians(args.train, args.nmin, args.nmax, args.tmin, args.tmax, args.seed)
     eval_h = hamiltonians(args.eval, args.eval_n, args.eval_n, args.tmin, args.tmax, args.seed + 1)
     log(f'{len(train_h)} training / {len(eval_h)} held-out Hamiltonians')

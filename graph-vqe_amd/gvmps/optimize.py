@@ -143,14 +143,25 @@ class _RoundRobin:
                 self._serve()
 
 
-def lbfgs_many(problems, x0s, maxiter=200, ftol=1e-9, gtol=1e-3, maxls=40, maxcor=10):
+class _ImmediateStop(Exception):
+    pass
+
+
+def lbfgs_many(problems, x0s, maxiter=200, ftol=1e-9, gtol=1e-3, maxls=40, maxcor=10, immediate_tol=None):
     """scipy L-BFGS-B on every (problem, x0) at once, with exact adjoint
     gradients: each optimizer runs in its own thread, and all their requests
     of one round are evaluated by one batched reverse pass.  Every iterate
     equals that of a sequential scipy run (the batched values are bitwise the
-    single-instance values).  Returns (results, rounds)."""
+    single-instance values).
+
+    immediate_tol: the campaign adapter's immediate energy stop
+    (test/tools/lbfgs_optimizer.py:64-67, 98-117, 172-186): a run ends as
+    soon as two consecutive objective evaluations differ by less than the
+    tolerance, and returns the best point evaluated.  Every result carries
+    `energies` (every objective evaluation) and `iterate_energies` (the start,
+    then each accepted iterate).  Returns (results, rounds)."""
     import threading
-    from scipy.optimize import minimize
+    from scipy.optimize import OptimizeResult, minimize
     from .engine import Batch
     batch = Batch(list(problems), [1] * len(problems))
     rr = _RoundRobin(batch, x0s)
@@ -158,10 +169,38 @@ def lbfgs_many(problems, x0s, maxiter=200, ftol=1e-9, gtol=1e-3, maxls=40, maxco
     errors = []
 
     def run(i):
+        x0 = np.asarray(x0s[i], dtype=float)
+        st = dict(prev=None, best=(np.inf, x0.copy()), evals=[], iters=[], nit=0)
+
+        def fg(x):
+            e, g = rr.request(i, x)
+            st['evals'].append(e)
+            if e < st['best'][0]:
+                st['best'] = (e, np.asarray(x, dtype=float).copy())
+            if not st['iters']:
+                st['iters'].append(e)
+            delta = None if st['prev'] is None else abs(e - st['prev'])
+            st['prev'] = e
+            if immediate_tol is not None and delta is not None and delta < immediate_tol:
+                raise _ImmediateStop()
+            return e, g
+
+        def accepted(xk):
+            st['nit'] += 1
+            st['iters'].append(st['evals'][-1])
+
         try:
-            results[i] = minimize(lambda x: rr.request(i, x), np.asarray(x0s[i], dtype=float), jac=True,
-                                  method='L-BFGS-B',
-                                  options=dict(maxiter=maxiter, ftol=ftol, gtol=gtol, maxls=maxls, maxcor=maxcor))
+            try:
+                res = minimize(fg, x0, jac=True, method='L-BFGS-B', callback=accepted,
+                               options=dict(maxiter=maxiter, ftol=ftol, gtol=gtol, maxls=maxls, maxcor=maxcor))
+                res.energies, res.iterate_energies = st['evals'], st['iters']
+            except _ImmediateStop:
+                res = OptimizeResult(x=st['best'][1], fun=st['best'][0], nit=st['nit'], nfev=len(st['evals']),
+                                     njev=len(st['evals']), success=True,
+                                     message=f'Immediate energy stopping: consecutive |delta E| < {immediate_tol:g} '
+                                             '(no patience/window)',
+                                     energies=st['evals'], iterate_energies=st['iters'])
+            results[i] = res
         except Exception as ex:
             errors.append(ex)
         finally:
