@@ -259,6 +259,20 @@ def main():
     torch.cuda.synchronize()
     dbl_rate = B * args.steps / (time.perf_counter() - t2)
     dbl_dev = float(torch.max(torch.abs(e_dbl - e) / torch.clamp(torch.abs(e), min=1.0)))
+    # the same energies with the pivoted QR / SVD run at every gate, although
+    # no split of this workload cuts rank (the default takes the exact split)
+    batch.set_prep('factorize')
+    e_fac = batch.energies(x)
+    torch.cuda.synchronize()
+    batch.set_timing(True)
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        e_fac = batch.energies(x)
+    torch.cuda.synchronize()
+    fac_rate = B * args.steps / (time.perf_counter() - t3)
+    fac_ms, fac_calls = batch.timing()
+    batch.set_timing(False)
+    fac_dev = float(torch.max(torch.abs(e_fac - e) / torch.clamp(torch.abs(e), min=1.0)))
     batch.set_prep('svd')
 
     grad = None if args.no_grad else bench_grads(args, prob, info, x, B)
@@ -283,6 +297,10 @@ def main():
         "plain_matches_batched_bitwise": same,
         "doubling_prep": {"evals_per_s": round(dbl_rate, 2), "max_rel_diff_vs_svd": dbl_dev,
                           "note": "same energies, state built by exact CNOT bond doubling instead of SVDs"},
+        "factorized_prep": {"evals_per_s": round(fac_rate, 2), "max_rel_diff": fac_dev,
+                            "state_prep_ms": round(fac_ms[0] / max(fac_calls, 1), 3),
+                            "note": "same energies, pivoted QR (SVD gauge) at every gate instead of the exact split"},
+        "exact_split": bool(info['exact_split']),
         "phase_ms_per_step": {"state_prep": round(phase_ms[0] / max(calls, 1), 3),
                               "forest_chains": round(chains_ms, 3),
                               "chain_kernel": round(kern_ms_step, 3),

@@ -203,6 +203,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.rot_scale = upload(B->owned, P.rot_scale.data(), P.rot_scale.size());
     d.nleaf = (int)P.leaves.size();
     d.leaves = upload(B->owned, P.leaves.data(), P.leaves.size());
+    d.exact_split = P.exact_split ? 1 : 0;
   }
   B->h_probs = pd;
   B->d_probs = upload(B->owned, pd.data(), pd.size());
@@ -389,8 +390,8 @@ void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
     hip_check(kl_build_local(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.arena, nullptr, B.status),
               "k_build_local launch");
   else
-    hip_check(kl_prep(ninst, B.real && B.arith == 0, stream, B.d_inst, B.d_probs, params, B.arena, B.bonds, B.status,
-                      B.stats),
+    hip_check(kl_prep(ninst, B.real && B.arith == 0, B.prep_mode == 2, stream, B.d_inst, B.d_probs, params, B.arena,
+                      B.bonds, B.status, B.stats),
               "k_prep launch");
   B.mark(1, stream);
 }
@@ -715,6 +716,7 @@ int gvmps_problem_get_info(const gvmps_problem *p, gvmps_problem_info *info) {
     info->grad_ok = P.grad_error.empty() ? 1 : 0;
     info->real_arith = P.real_ok ? 1 : 0;
     info->iz_forest = P.iz_only ? 1 : 0;
+    info->exact_split = P.exact_split ? 1 : 0;
   })
 }
 
@@ -924,7 +926,9 @@ int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride) {
 int gvmps_batch_set_prep(gvmps_batch *b, int mode) {
   GV_TRY({
     Batch &B = *reinterpret_cast<Batch *>(b);
-    if (mode != 0 && mode != 1) throw std::runtime_error("preparation mode must be 0 (SVD) or 1 (bond doubling)");
+    if (mode < 0 || mode > 2)
+      throw std::runtime_error("preparation mode must be 0 (gates + splits), 1 (bond doubling) or 2 (gates + a "
+                               "decomposition at every split)");
     if (mode == 1)
       for (const Problem *P : B.probs)
         if (!P->grad_error.empty()) throw std::runtime_error(P->grad_error);

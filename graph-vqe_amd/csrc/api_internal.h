@@ -52,7 +52,8 @@ struct Batch {
   // launch configuration of the chain kernels (set by the autotuner)
   int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
   int order = 1;  // 0: instance-major items, 1: longest chains first within each XCD's share
-  int prep_mode = 0;  // 0: gates + SVDs (k_prep), 1: exact bond doubling (k_build_local)
+  int prep_mode = 0;  // 0: gates + splits (k_prep), 1: exact bond doubling (k_build_local),
+                      // 2: gates + a decomposition at every split, exact ones included
   // real-arithmetic energies: every problem has real gates and no Y words
   bool real = false;
   bool iz = false;    // every problem's forest has I/Z children only: k_rchains*_iz
@@ -107,7 +108,7 @@ void launch_prepare(Batch &B, const double *params, hipStream_t stream);
 void launch_energy_tail(Batch &B, hipStream_t stream, bool terms);
 
 // kernel launchers (kernels_energy.hip)
-hipError_t kl_prep(int ninst, bool real, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_prep(int ninst, bool real, bool factor_always, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                    const double *params, double2 *arena, int *bonds, int *status, int *stats);
 hipError_t kl_envs(int ninst, hipStream_t s, const InstDev *insts, const double2 *arena, double2 *envs);
 hipError_t kl_transpose_sites(long long n, hipStream_t s, double2 *arena);

@@ -156,6 +156,7 @@ struct ProbDev {
   const int* param_rot_begin;   // nparams+1 offsets: rotations driven by each parameter
   const int* param_rots;
   const double* rot_scale;      // per rotation: d angle / d parameter
+  int exact_split;              // 1: every two-site split keeps all of Theta (zero cutoffs, no rank reduction)
 };
 
 // One MPS instance of a batch.

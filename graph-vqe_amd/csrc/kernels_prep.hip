@@ -20,6 +20,10 @@
 //      zero-cutoff guard (selectDirectSvdRank, dist_svd.cuh:194-212);
 //      left site <- Q[:, :k] (isometry), right site <- W[:k, :] = S_k Vh_k
 //      (packDirectSvdFactors, dist_svd.cuh:214-231).
+// When the problem's cutoffs are zero and no gate of the tape is cut below
+// min(2 Dl, 2 Dr) (exact_split), steps 3-4 are replaced by the exact split
+// Theta = I Theta or Theta I (run_gate), unless the batch asks for the
+// factorisation at every gate.
 #include "device_common.h"
 
 using namespace gv;
@@ -375,7 +379,7 @@ __device__ int wave_jacobi(double2 (*W)[PW], int rows, int nc, int width, int la
 // product instead of four and the QR runs in real arithmetic.
 template <bool REAL>
 __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const GateDev gd, const double *x,
-                         double2 *sites, int *status) {
+                         double2 *sites, int *status, bool exact) {
   const unsigned long long t0 = clk();
   const int s = gd.site;
   const int Dl = L.bond[s], Dc = L.bond[s + 1], Dr = L.bond[s + 2];
@@ -445,13 +449,43 @@ __device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const Ga
         }
     }
   }
-  for (int idx = lane; idx < m * m; idx += 64) {
-    const int row = idx / m, col = idx % m;
-    W[row][nc + col] = make_double2(row == col ? 1.0 : 0.0, 0.0);
-  }
+  if (!exact)
+    for (int idx = lane; idx < m * m; idx += 64) {
+      const int row = idx / m, col = idx % m;
+      W[row][nc + col] = make_double2(row == col ? 1.0 : 0.0, 0.0);
+    }
   frob = wave_sum_uniform(frob);
   wave_sync();
   const unsigned long long t1 = clk();
+
+  if (exact) {
+    // Exact split: zero cutoffs and no gate of the tape cut below
+    // min(2 Dl, 2 Dr), so the rule keeps k = min(m, nc) columns, all of
+    // Theta.  Theta = I_m Theta (m <= nc: left site the identity isometry,
+    // right site Theta's rows) or Theta = Theta I_nc (nc < m: left site
+    // Theta, right site the identity co-isometry) is then the same state as
+    // U_k (S_k Vh_k) with no rounding at all, and no singular value is
+    // compared with anything, so no decomposition is needed.
+    const bool left_id = m <= nc;
+    const int k = left_id ? m : nc;
+    double2 *wa = sites + (size_t)s * GV_SLOT;
+    double2 *wb = sites + (size_t)(s + 1) * GV_SLOT;
+    for (int idx = lane; idx < GV_SLOT; idx += 64) {
+      const int row = idx >> 5, I = (idx >> 4) & 1, c = idx & 15;
+      double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+      if (row < Dl && c < k) va = left_id ? make_double2(c == row * 2 + I ? 1.0 : 0.0, 0.0) : W[row * 2 + I][c];
+      if (row < k && c < Dr) vb = left_id ? W[row][I * Dr + c] : make_double2(row == I * Dr + c ? 1.0 : 0.0, 0.0);
+      wa[idx] = va;
+      wb[idx] = vb;
+    }
+    if (lane == 0) {
+      if (!isfinite(frob) && status) atomicMax(status, ST_NONFINITE);
+      L.bond[s + 1] = k;
+      atomicAdd(&L.cyc[0], (unsigned int)(t1 - t0));
+      atomicAdd(&L.cyc[3], (unsigned int)(clk() - t1));
+    }
+    return;
+  }
 
   // Preconditioning: pivoted QR leaves the singular values in the first
   // `rp` rows when the trailing block R22 is below the discard scale of the
@@ -564,7 +598,7 @@ template <bool REAL>
 __device__ __forceinline__ void prep_body(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
                                           const double *__restrict__ params, double2 *__restrict__ arena,
                                           int *__restrict__ bonds_out, int *__restrict__ status,
-                                          int *__restrict__ stats) {
+                                          int *__restrict__ stats, int factor_always) {
   __shared__ PrepLds L;
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   const InstDev inst = uload(insts + blockIdx.x);
@@ -584,9 +618,10 @@ __device__ __forceinline__ void prep_body(const InstDev *__restrict__ insts, con
   }
   __syncthreads();
 
+  const bool exact = P.exact_split && !factor_always;
   for (int row = 0; row < P.nsched; ++row) {
     const int gi = uload(P.sched + row * GV_WAVES + w);
-    if (gi >= 0) run_gate<REAL>(L, w, lane, P, uload(P.g2 + gi), x, sites, st);
+    if (gi >= 0) run_gate<REAL>(L, w, lane, P, uload(P.g2 + gi), x, sites, st, exact);
     __syncthreads();
   }
   // single-site ops left pending after the last two-site gate on each site
@@ -614,28 +649,29 @@ __device__ __forceinline__ void prep_body(const InstDev *__restrict__ insts, con
 
 extern "C" __global__ void __launch_bounds__(64 * GV_WAVES)
 k_prep(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
-       double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status, int *__restrict__ stats) {
-  prep_body<false>(insts, probs, params, arena, bonds_out, status, stats);
+       double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status, int *__restrict__ stats,
+       int factor_always) {
+  prep_body<false>(insts, probs, params, arena, bonds_out, status, stats, factor_always);
 }
 
 // Real circuits (every gate matrix real): the same preparation in real arithmetic.
 extern "C" __global__ void __launch_bounds__(64 * GV_WAVES)
 k_prep_real(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
             double2 *__restrict__ arena, int *__restrict__ bonds_out, int *__restrict__ status,
-            int *__restrict__ stats) {
-  prep_body<true>(insts, probs, params, arena, bonds_out, status, stats);
+            int *__restrict__ stats, int factor_always) {
+  prep_body<true>(insts, probs, params, arena, bonds_out, status, stats, factor_always);
 }
 
 namespace gv {
 
-hipError_t kl_prep(int ninst, bool real, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+hipError_t kl_prep(int ninst, bool real, bool factor_always, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                    const double *params, double2 *arena, int *bonds, int *status, int *stats) {
   if (real)
     hipLaunchKernelGGL(k_prep_real, dim3(ninst), dim3(64 * GV_WAVES), 0, s, insts, probs, params, arena, bonds,
-                       status, stats);
+                       status, stats, (int)factor_always);
   else
     hipLaunchKernelGGL(k_prep, dim3(ninst), dim3(64 * GV_WAVES), 0, s, insts, probs, params, arena, bonds, status,
-                       stats);
+                       stats, (int)factor_always);
   return hipGetLastError();
 }
 

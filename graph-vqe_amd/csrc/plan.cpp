@@ -117,6 +117,7 @@ struct CircuitCompiler {
     P.prog.push_back(op);
     const int m = 2 * bond[s], nc = 2 * bond[s + 2];
     const int k = std::min(std::min(m, nc), P.max_bond);
+    if (k < std::min(m, nc)) P.rank_reduced = true;
     bond[s + 1] = k;
     P.max_alloc_bond = std::max(P.max_alloc_bond, k);
     P.n_g2++;
@@ -755,6 +756,12 @@ std::unique_ptr<Problem> compile_problem(int n_qubits, const gvmps_op *ops, int 
   if (P->max_alloc_bond > GV_CHI)
     throw std::runtime_error("allocated bond " + std::to_string(P->max_alloc_bond) +
                              " exceeds the 16-wide bond slots of the GPU layout");
+  // With zero cutoffs the rank rule keeps min(2 Dl, 2 Dr, max_bond) columns
+  // (dist_svd.cuh:194-212).  When no gate of the tape is cut below
+  // min(2 Dl, 2 Dr), every split keeps the whole of Theta, no singular value
+  // is compared with anything (the guard needs max_bond < count), and the
+  // state does not depend on which exact factorisation is used.
+  P->exact_split = abs_cut == 0.0 && rel_cut == 0.0 && !P->rank_reduced;
   build_schedule(*P);
   build_local_programs(*P, ops, n_ops);
   P->nterms = n_terms;

@@ -22,6 +22,8 @@ struct Problem {
   std::vector<double> fixes;   // 8 doubles per fixed single-site gate
   std::vector<int> bonds_pred; // n+1, zero-cutoff allocation
   int max_alloc_bond = 1, n_g2 = 0;
+  bool rank_reduced = false;   // some two-site gate has min(2 Dl, 2 Dr) > max_bond
+  bool exact_split = false;    // zero cutoffs and no rank reduction: every split is exact
   // pipelined schedule of the two-site gates
   std::vector<GateDev> g2;
   std::vector<int> sched;      // rows of GV_WAVES gate indices (-1 idle)

@@ -29,7 +29,8 @@ class ProblemInfo(ctypes.Structure):
         ('flops_svd_nominal', ctypes.c_double), ('flops_grad_executed', ctypes.c_double),
         ('n_rotations', ctypes.c_int32), ('grad_ok', ctypes.c_int32),
         ('real_arith', ctypes.c_int32),
-        ('iz_forest', ctypes.c_int32)]
+        ('iz_forest', ctypes.c_int32),
+        ('exact_split', ctypes.c_int32)]
 
 
 # (name, argtypes); every function returns int status except where noted

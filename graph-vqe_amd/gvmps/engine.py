@@ -167,9 +167,12 @@ class Batch:
         check(lib().gvmps_batch_set_checkpoint(self._h, int(stride)))
 
     def set_prep(self, mode):
-        """'svd' (gates + exact-bond SVD, the plugin's method) or 'doubling'
-        (exact CNOT bond doubling, no SVD; needs a doubling-compatible tape)."""
-        m = {'svd': 0, 'doubling': 1}[mode]
+        """'svd' (gates + the plugin's exact-bond split: a decomposition
+        wherever the rank rule can cut, the exact split Theta = I Theta or
+        Theta I where no gate of the problem is cut), 'factorize' (a
+        decomposition at every gate) or 'doubling' (exact CNOT bond doubling,
+        no SVD; needs a doubling-compatible tape)."""
+        m = {'svd': 0, 'doubling': 1, 'factorize': 2}[mode]
         check(lib().gvmps_batch_set_prep(self._h, m))
         self.prep = mode
 

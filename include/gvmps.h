@@ -56,6 +56,8 @@ typedef struct gvmps_problem_info {
   int32_t grad_ok;                 /* 1: the tape has a bond-doubling reverse gradient */
   int32_t real_arith;              /* 1: real gate matrices and no Y words (real-arithmetic energies) */
   int32_t iz_forest;               /* 1: no X or Y words: every forest step has I/Z children only */
+  int32_t exact_split;             /* 1: zero cutoffs and no two-site gate cut below min(2 Dl, 2 Dr):
+                                      every split keeps all of Theta and needs no decomposition */
 } gvmps_problem_info;
 
 /* Last error message of the calling thread ("" if none). */
@@ -165,14 +167,19 @@ int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride);
 int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order);
 
 /* State-preparation method of the batch's energy, sample and evaluate calls:
- * 0 (default) applies the gates with the exact-bond SVD, as the plugin does
- * (sharded_mps.cuh:465-572, dist_svd.cuh:194-283); 1 builds the same state
- * by exact CNOT bond doubling, the representation of the reference's reverse
- * engine (test/tools/reverse_mps.py:45-50), with no SVD.  Mode 1 needs every
- * problem's tape to have a doubling form within 16-wide bonds (grad_ok);
- * energies and probabilities are those of the same state (gauge-invariant),
- * equal to mode 0 within rounding.  read_sites then returns doubling tensors
- * and no bond profile. */
+ * 0 (default) applies the gates with the exact-bond split of the plugin
+ * (sharded_mps.cuh:465-572, dist_svd.cuh:194-283): the pivoted QR / Jacobi
+ * SVD and the rank rule at every gate, except for problems whose splits all
+ * keep the whole of Theta (exact_split in the problem info: zero cutoffs and
+ * no gate cut below min(2 Dl, 2 Dr)), which take the exact split
+ * Theta = I Theta or Theta I with no decomposition; 2 runs the decomposition
+ * at every gate even then; 1 builds the same state by exact CNOT bond
+ * doubling, the representation of the reference's reverse engine
+ * (test/tools/reverse_mps.py:45-50), with no SVD.  Mode 1 needs every
+ * problem's tape to have a doubling form within 16-wide bonds (grad_ok).
+ * Energies and probabilities are gauge-invariant: all modes give them for
+ * the same state, equal within rounding.  read_sites returns the mode's own
+ * tensors (no bond profile in mode 1). */
 int gvmps_batch_set_prep(gvmps_batch *b, int mode);
 
 /* Arithmetic of state preparation, energy forests and reverse gradients.

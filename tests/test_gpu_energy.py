@@ -312,3 +312,52 @@ def test_bond_doubling_preparation_matches_svd(cuda):
     b0.set_prep('doubling')
     bits_dbl, _ = b0.sample(w0['x'], 1000, [12345])
     assert np.array_equal(bits_svd.cpu().numpy(), bits_dbl.cpu().numpy())
+
+
+def test_exact_split_matches_factorisation(cuda):
+    """Problems whose splits all keep the whole of Theta (zero cutoffs, no
+    rank reduction) skip the decomposition (Theta = I Theta or Theta I).
+    The state is the same as with the pivoted QR / SVD at every gate
+    (preparation 'factorize'): energies agree to rounding, on the real and
+    the complex kernels, with the oracle too, and the sampled bitstrings are
+    the same."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    assert p.info()['exact_split'] == 1
+    X = w['x'][:8]
+    print('case: spsa-40q exact split vs factorisation, real and complex kernels', flush=True)
+    b = Batch([p], [8])
+    for arith in ('auto', 'complex'):
+        b.set_arith(arith)
+        b.set_prep('svd')
+        e_exact = b.energies(X).cpu().numpy()
+        b.set_prep('factorize')
+        e_fact = b.energies(X).cpu().numpy()
+        assert rel_close(e_exact, e_fact, 1e-12), (arith, e_exact - e_fact)
+    for i in (0, 5):
+        eo, _ = oracle_terms(w['n'], w['ops'], X[i], w['words'], w['coeffs'])
+        assert rel_close(e_exact[i], eo)
+    r = np.random.default_rng(41)
+    n = 10
+    ops, P = workloads.su2_ansatz(n, 3, ('ry', 'rz'), 'reverse_linear')
+    words = [''.join(r.choice(list('IXYZ'), n)) for _ in range(60)] + ['I' * n]
+    coeffs = r.normal(size=len(words))
+    Xc = r.uniform(-np.pi, np.pi, (3, P))
+    q = Problem(n, ops, P, words, coeffs, max_bond=16, abs_cut=0, rel_cut=0)
+    assert q.info()['exact_split'] == 1 and q.info()['real_arith'] == 0
+    print('case: 10q complex circuit, X/Y/Z words, exact split vs factorisation vs dense', flush=True)
+    bq = Batch([q], [3])
+    e1 = bq.energies(Xc).cpu().numpy()
+    bq.set_prep('factorize')
+    e2 = bq.energies(Xc).cpu().numpy()
+    for i in range(3):
+        d = dense.Dense(n).run(tape_with_values(ops, Xc[i])).energy(words, coeffs)
+        assert rel_close(e1[i], d) and rel_close(e2[i], d)
+    w0 = workloads.config(0)
+    p0 = Problem(w0['n'], w0['ops'], w0['nparams'], w0['words'], w0['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    print('case: 12q samples, exact split vs factorisation', flush=True)
+    b0 = Batch([p0], [1])
+    bits_exact, _ = b0.sample(w0['x'], 1000, [9876])
+    b0.set_prep('factorize')
+    bits_fact, _ = b0.sample(w0['x'], 1000, [9876])
+    assert np.array_equal(bits_exact.cpu().numpy(), bits_fact.cpu().numpy())

@@ -89,3 +89,22 @@ def test_forest_dedup_counts():
     info = p.info()
     assert info['n_active_terms'] == 3
     assert info['n_steps'] <= 3 * n
+
+
+def test_exact_split_rule():
+    """exact_split: zero cutoffs and no two-site gate cut below
+    min(2 Dl, 2 Dr) by max_bond (then every split keeps all of Theta)."""
+    from gvmps import Problem, workloads
+    for i in (0, 1, 2):
+        w = workloads.config(i)
+        p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+        assert p.info()['exact_split'] == 1
+    ops, P = workloads.su2_ansatz(10, 3, ('ry', 'rz'), 'linear')
+    words = ['Z' * 10]
+    assert Problem(10, ops, P, words, [1.0], max_bond=64, abs_cut=0, rel_cut=0).info()['exact_split'] == 1
+    # cutoffs compare singular values with a threshold at every gate
+    assert Problem(10, ops, P, words, [1.0], max_bond=64, abs_cut=1e-5, rel_cut=0).info()['exact_split'] == 0
+    assert Problem(10, ops, P, words, [1.0], max_bond=64, abs_cut=0, rel_cut=1e-5).info()['exact_split'] == 0
+    # a bond cap below the circuit's allocation bond cuts rank (and arms the guard)
+    assert Problem(10, ops, P, words, [1.0], max_bond=4, abs_cut=0, rel_cut=0).info()['exact_split'] == 0
+    assert Problem(10, ops, P, words, [1.0], max_bond=8, abs_cut=0, rel_cut=0).info()['exact_split'] == 1
