@@ -476,11 +476,29 @@ unsigned long long *launch_sample(Batch &B, const double *params, int shots, con
   }
   hip_check(hipMemcpyAsync(B.seeds, seeds, sizeof(unsigned) * ninst, hipMemcpyHostToDevice, stream), "seeds H2D");
   launch_prepare(B, params, stream);
-  hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
-  B.mark(2, stream);
-  hip_check(kl_sample(ninst, shots, max_n, stream, B.d_inst, B.arena, B.envs, B.seeds, bits, packed), "k_sample launch");
-  B.mark(3, stream);
+  launch_sample_tail(B, shots, B.seeds, bits, packed, stream, true);
   return packed;
+}
+
+// Right environments and the shot sweep of a prepared batch.  Real batches
+// (every problem real, real arithmetic selected) run the real kernels on the
+// packed real arena, packed here unless the energy tail already did.
+void launch_sample_tail(Batch &B, int shots, const unsigned *seeds_dev, unsigned char *bits,
+                        unsigned long long *packed, hipStream_t stream, bool pack) {
+  const int ninst = (int)B.inst.size();
+  if (B.real && B.arith == 0) {
+    if (pack) hip_check(kl_pack_real(B.n_site_total, stream, B.arena, B.rarena), "k_pack_real launch");
+    hip_check(kl_renvs_right(ninst, stream, B.d_inst, B.rarena, B.n_site_total, B.envs), "k_renvs_right launch");
+    B.mark(2, stream);
+    hip_check(kl_sample_real(ninst, shots, B.max_n, stream, B.d_inst, B.rarena, B.envs, seeds_dev, bits, packed),
+              "k_sample_real launch");
+  } else {
+    hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
+    B.mark(2, stream);
+    hip_check(kl_sample(ninst, shots, B.max_n, stream, B.d_inst, B.arena, B.envs, seeds_dev, bits, packed),
+              "k_sample launch");
+  }
+  B.mark(3, stream);
 }
 
 // Energies and, for shots > 0, samples of every instance from one state
@@ -496,11 +514,8 @@ void launch_evaluate(Batch &B, const double *params, int shots, const unsigned *
   if (energies_dev)
     hip_check(hipMemcpyAsync(energies_dev, B.energies, sizeof(double) * ninst, hipMemcpyDeviceToDevice, stream),
               "copy energies");
-  if (shots > 0) {
-    hip_check(kl_envs(ninst, stream, B.d_inst, B.arena, B.envs), "k_envs launch");
-    hip_check(kl_sample(ninst, shots, B.max_n, stream, B.d_inst, B.arena, B.envs, seeds_dev, bits, packed),
-              "k_sample launch");
-  }
+  // the energy tail packed the real arena when the batch is real
+  if (shots > 0) launch_sample_tail(B, shots, seeds_dev, bits, packed, stream, false);
   B.mark(5, stream);
 }
 
@@ -814,13 +829,9 @@ int gvmps_batch_shot_energy_noalloc(gvmps_batch *b, const double *params_dev, in
     unsigned long long *packed = scratch_address(B.device);
     hip_check(hipMemcpyAsync(B.seeds, seeds, sizeof(unsigned) * ninst, hipMemcpyHostToDevice, s), "seeds H2D");
     launch_prepare(B, params_dev, s);
-    hip_check(kl_envs(ninst, s, B.d_inst, B.arena, B.envs), "k_envs launch");
-    B.mark(2, s);
     // every live shot's two words are written before k_zsums reads them, so
     // nothing in the scratch needs clearing
-    hip_check(kl_sample(ninst, shots, B.max_n, s, B.d_inst, B.arena, B.envs, B.seeds, nullptr, packed),
-              "k_sample launch");
-    B.mark(3, s);
+    launch_sample_tail(B, shots, B.seeds, nullptr, packed, s, true);
     hip_check(kl_zsums(ninst, B.max_terms, shots, s, B.d_inst, B.d_probs, B.d_masks, packed, B.terms), "k_zsums");
     hip_check(kl_reduce(ninst, s, B.d_inst, B.d_probs, B.terms, B.energies), "k_reduce launch");
     B.mark(4, s);

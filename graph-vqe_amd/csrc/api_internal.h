@@ -106,6 +106,8 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
 void check_status(Batch &B, hipStream_t stream);
 void launch_prepare(Batch &B, const double *params, hipStream_t stream);
 void launch_energy_tail(Batch &B, hipStream_t stream, bool terms);
+void launch_sample_tail(Batch &B, int shots, const unsigned *seeds_dev, unsigned char *bits,
+                        unsigned long long *packed, hipStream_t stream, bool pack);
 
 // kernel launchers (kernels_energy.hip)
 hipError_t kl_prep(int ninst, bool real, bool factor_always, hipStream_t s, const InstDev *insts, const ProbDev *probs,
@@ -157,6 +159,10 @@ hipError_t kl_param_grad(int ninst, int max_params, hipStream_t s, const InstDev
 // kernel launchers (kernels_sample.hip)
 hipError_t kl_sample(int ninst, int shots, int stride, hipStream_t s, const InstDev *insts, const double2 *arena,
                      const double2 *envs, const unsigned *seeds, unsigned char *bits, unsigned long long *packed);
+hipError_t kl_renvs_right(int ninst, hipStream_t s, const InstDev *insts, const double *rarena, long long nsite,
+                          double2 *envs);
+hipError_t kl_sample_real(int ninst, int shots, int stride, hipStream_t s, const InstDev *insts, const double *rarena,
+                          const double2 *envs, const unsigned *seeds, unsigned char *bits, unsigned long long *packed);
 hipError_t kl_zsums(int ninst, int max_terms, int shots, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                     const unsigned long long *const *masks, const unsigned long long *packed, double2 *terms);
 

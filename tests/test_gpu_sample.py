@@ -112,3 +112,45 @@ def test_six_patterns_shot_error_bound(cuda):
         shot = b.shot_energies(x, 8192, [rng.shot_shard_seed(rng.INITIAL_SAMPLE_SEED, 0)]).cpu().numpy()[0]
         assert abs(shot - exact) < 0.08
         assert abs(exact - dense.Dense(n).run(tapev(ops, x)).energy(words, coeffs)) < 1e-10
+
+
+def test_real_sampling_kernels_match_complex_and_oracle(cuda):
+    """Real circuits sample on the real kernels (real right environments and
+    shot vectors on the packed real arena).  Their bits equal the complex
+    kernels' bit for bit (the complex kernels only add exact zeros to the
+    same accumulations) and the oracle's draws, on the 12-qubit
+    configuration and on the 16 largest uneven partitions (26-32 qubits),
+    through sample, evaluate and the shot-energy path."""
+    w = workloads.config(0)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    assert p.info()['real_arith'] == 1
+    seed = rng.shot_shard_seed(rng.INITIAL_SAMPLE_SEED + 3, 0)
+    print('case: 12q real vs complex sampling kernels', flush=True)
+    b = Batch([p], [1])
+    bits_r, packed_r = b.sample(w['x'], 1000, [seed])
+    e_r = b.shot_energies(w['x'], 1000, [seed]).cpu().numpy()
+    b.set_arith('complex')
+    bits_c, packed_c = b.sample(w['x'], 1000, [seed])
+    e_c = b.shot_energies(w['x'], 1000, [seed]).cpu().numpy()
+    assert np.array_equal(bits_r.cpu().numpy(), bits_c.cpu().numpy())
+    assert np.array_equal(packed_r.cpu().numpy(), packed_c.cpu().numpy()) and np.array_equal(e_r, e_c)
+    sim = mps_ref.RefSimulator(w['n'], 16, 0.0, 0.0).run(tapev(w['ops'], w['x'][0]))
+    ref, margins = sim.mps.sample_bits(1000, seed, margins=True)
+    assert assert_same_bits(bits_r.cpu().numpy()[0], ref, margins) == 0
+    parts = sorted(workloads.config(3)['parts'], key=lambda q: -q['n'])[:16]
+    probs = [Problem(q['n'], q['ops'], q['nparams'], q['words'], q['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+             for q in parts]
+    x = np.concatenate([q['x'][0] for q in parts])
+    sd = [rng.shot_shard_seed(2000 + j, 0) for j in range(len(parts))]
+    print('case: 16 largest partitions, evaluate on real vs complex kernels', flush=True)
+    bb = Batch(probs, [1] * len(parts))
+    e1, b1, _ = bb.evaluate(x, 1000, sd)
+    bb.set_arith('complex')
+    e2, b2, _ = bb.evaluate(x, 1000, sd)
+    assert np.array_equal(b1.cpu().numpy(), b2.cpu().numpy())
+    assert np.all(np.abs(e1.cpu().numpy() - e2.cpu().numpy()) <= 1e-12 * np.abs(e2.cpu().numpy()))
+    b1 = b1.cpu().numpy()
+    for j, q in enumerate(parts[:4]):
+        sim = mps_ref.RefSimulator(q['n'], 16, 0.0, 0.0).run(tapev(q['ops'], q['x'][0]))
+        ref, margins = sim.mps.sample_bits(1000, sd[j], margins=True)
+        assert assert_same_bits(b1[j, :, :q['n']], ref, margins) == 0
