@@ -6,7 +6,8 @@ no side chains, Miyazawa-Jernigan pair energies and penalty parameters
 (10, 10, 10), compressed to the qubits it uses.  Every operator of that
 construction is diagonal, a sum of Z-strings, so it is restated here on an
 ordered list of (Z-string, coefficient) terms; a Z-string is an integer bit
-mask, qubit q = bit q (Qiskit's order).  The steps follow
+mask, qubit q = bit q (Qiskit's order).  The steps follow the package the
+campaign imports (test/sources/independent-vqe/, paths below relative to it)
 
   Protein_Folding/peptide/chains/base_chain.py:58-75    turn qubits (I + Z_k) / 2
   Protein_Folding/peptide/beads/main_bead.py:59-74      turn indicator functions
