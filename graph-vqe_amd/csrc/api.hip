@@ -198,6 +198,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.seed_terms = upload(B->owned, P.seed_terms.data(), P.seed_terms.size());
     d.site_steps_begin = upload(B->owned, P.site_steps_begin.data(), P.site_steps_begin.size());
     d.site_steps = upload(B->owned, P.site_steps.data(), P.site_steps.size());
+    d.site_tail = upload(B->owned, P.site_tail.data(), P.site_tail.size());
     d.param_rot_begin = upload(B->owned, P.param_rot_begin.data(), P.param_rot_begin.size());
     d.param_rots = upload(B->owned, P.param_rots.data(), P.param_rots.size());
     d.rot_scale = upload(B->owned, P.rot_scale.data(), P.rot_scale.size());
@@ -597,7 +598,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.mark(4, stream);
     hip_check(kl_rsite_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_rsite_reduce launch");
     B.mark(5, stream);
-    hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad),
+    hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad, 1),
               "k_site_bwd launch");
     hip_check(kl_param_grad(ninst, B.max_params, stream, B.d_inst, B.d_probs, B.rgrad, B.grads),
               "k_param_grad launch");
@@ -630,7 +631,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   B.mark(4, stream);
   hip_check(kl_site_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_site_reduce launch");
   B.mark(5, stream);
-  hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad),
+  hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad, 0),
             "k_site_bwd launch");
   hip_check(kl_param_grad(ninst, B.max_params, stream, B.d_inst, B.d_probs, B.rgrad, B.grads),
             "k_param_grad launch");

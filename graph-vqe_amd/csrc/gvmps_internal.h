@@ -157,6 +157,11 @@ struct ProbDev {
   const int* param_rots;
   const double* rot_scale;      // per rotation: d angle / d parameter
   int exact_split;              // 1: every two-site split keeps all of Theta (zero cutoffs, no rank reduction)
+  // Per site: the CNOT halves that end its local program, at most one per
+  // bond side, as count | (last op kind << 4) | (op before it << 8).  The real
+  // backward chains pull each step's site adjoint back through them before
+  // storing it, so the stored adjoint has 128, 256 or 512 entries.
+  const int* site_tail;
 };
 
 // One MPS instance of a batch.

@@ -515,10 +515,14 @@ k_site_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
 
 // Local program in reverse.  Rotation k: T_{k+1} = U T_k, so
 // dE/dtheta = Re sum conj(Tbar_{k+1}) (dU/dtheta T_k) and Tbar_k = U^H Tbar_{k+1};
-// CNOT halves pull the adjoint back through their index maps.
+// CNOT halves pull the adjoint back through their index maps.  With
+// pulled_tail != 0 the adjoint in abar has already been pulled back through
+// the site's trailing CNOT halves (ProbDev::site_tail), and the program
+// resumes before them.
 extern "C" __global__ void __launch_bounds__(64)
 k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
-           const double2 *__restrict__ abar, const double2 *__restrict__ rotin, double *__restrict__ rgrad) {
+           const double2 *__restrict__ abar, const double2 *__restrict__ rotin, double *__restrict__ rgrad,
+           int pulled_tail) {
   __shared__ double2 buf[2][GV_SLOT];
   const int lane = threadIdx.x;
   const InstDev inst = insts[blockIdx.y];
@@ -531,7 +535,7 @@ k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
   for (int k = 0; k < 8; ++k) buf[0][lane + 64 * k] = src[lane + 64 * k];
   __syncthreads();
   int cur = 0;
-  const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1];
+  const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1] - (pulled_tail ? (P.site_tail[s] & 3) : 0);
   for (int o = o1 - 1; o >= o0; --o) {
     const LocalOp op = P.lops[o];
     double2 U[4], dU[4];
@@ -629,8 +633,10 @@ hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *in
 }
 
 hipError_t kl_site_bwd(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad) {
-  hipLaunchKernelGGL(k_site_bwd, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, abar, rotin, rgrad);
+                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad,
+                       int pulled_tail) {
+  hipLaunchKernelGGL(k_site_bwd, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, abar, rotin, rgrad,
+                     pulled_tail);
   return hipGetLastError();
 }
 

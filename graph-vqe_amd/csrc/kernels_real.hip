@@ -274,14 +274,73 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
   for (int v = 0; v < 4; ++v) e.v[v] = kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
 }
 
+// Stores one step's site adjoint, given as rows kdx(g4, v), column a16 of
+// Mbar_0 (m0) and Mbar_1 (m1), pulled back through the CNOT halves that end
+// the site's local program (`tail`, see ProbDev::site_tail), in the compact
+// layout ((l * 2 + i) * rc + r) of the pulled-back dimensions: 16x2x16,
+// 16x2x8, 8x2x16 or 8x2x8.  The pull-backs (new bond bit = the index's lowest
+// bit, as in k_site_bwd):
+//   control, right:  Abar[l,i,r] = A'bar[l,i,2r+i]
+//   target,  right:  Abar[l,i,r] = A'bar[l,i,2r] + A'bar[l,i^1,2r+1]
+// and the same on rows for the left halves.  Row parity is g4 & 1 and column
+// parity a16 & 1, so the first pull-back leaves each lane one component i
+// (its column parity after a right half, its row parity after a left half),
+// pairing lanes 1 or 16 apart; a second one, on the other bond, pairs lanes
+// 17 apart and keeps half of the lanes.  The sums are the ones k_site_bwd
+// makes, per step instead of per site.
+__device__ __forceinline__ void radj_store(double *slot, int tail, int a16, int g4, const d4 &m0, const d4 &m1) {
+  const int nt = tail & 3;
+  if (nt == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      slot[sidx(kdx(g4, v), 0, a16)] = m0[v];
+      slot[sidx(kdx(g4, v), 1, a16)] = m1[v];
+    }
+    return;
+  }
+  const int k1 = (tail >> 4) & 15;
+  const bool r1 = k1 == LO_CTRL_R || k1 == LO_TARG_R;
+  const int c = r1 ? (a16 & 1) : (g4 & 1);
+  double y[4];
+  if (k1 == LO_TARG_R || k1 == LO_TARG_L) {
+    const int x = r1 ? 1 : 16;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = m0[v] + __shfl_xor(m1[v], x);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = c ? m1[v] : m0[v];
+  }
+  bool keep = true;
+  if (nt == 2) {
+    const int k2 = (tail >> 8) & 15;
+    if (k2 == LO_TARG_R || k2 == LO_TARG_L) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) y[v] += __shfl_xor(y[v], 17);
+      keep = ((r1 ? g4 : a16) & 1) == 0;
+    } else {
+      keep = (a16 & 1) == (g4 & 1);
+    }
+  }
+  const bool lh = !r1 || nt == 2, rh = r1 || nt == 2;
+  const int rc = rh ? 8 : 16, col = rh ? a16 >> 1 : a16;
+  if (keep) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = kdx(g4, v);
+      slot[((lh ? row >> 1 : row) * 2 + c) * rc + col] = y[v];
+    }
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------ reverse gradient, real
 // The bond-doubling state of a real tape (RY, CX, ...) is real, and so are
 // the forest environments of I/X/Z words and their adjoints (the adjoint of a
 // real symmetric environment is real symmetric: conj and transpose drop out
-// of the complex formulas).  Same slots as kernels_grad.hip; site adjoints use
-// the first 4 KiB of each 8 KiB step slot.  Hand-off spills, suffix stores,
+// of the complex formulas).  Same slots as kernels_grad.hip; a step's site
+// adjoint, pulled back through the CNOT halves that end its site's program
+// (radj_store), uses the first 1 to 4 KiB of its 8 KiB step slot.  Hand-off spills, suffix stores,
 // prefix leaves and adjoint spills are packed symmetric; the checkpointed
 // step environments stay full, so a recomputed step sees exactly the
 // environment the forward pass used and every checkpoint stride gives the
@@ -486,7 +545,7 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
       er = mfma(g1[t], fa.v[1][t], er);
     }
     // Mbar_i = 2 E G_i: G_i to the B-operand layout through LDS
-    double *slot = (double *)(as + (size_t)step * GV_SLOT);
+    d4 mb[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const d4 &g = i ? g1 : g0;
@@ -514,47 +573,62 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
 #pragma unroll
         for (int t = 0; t < 4; ++t) m = mfma(E.v[t], d[t], m);
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) slot[sidx(kdx(g4, v), i, a16)] = m[v];
+      mb[i] = m;
     }
+    radj_store((double *)(as + (size_t)step * GV_SLOT), uload(P.site_tail + sd.site), a16, g4, mb[0], mb[1]);
 #pragma unroll
     for (int t = 0; t < 4; ++t) eb.v[t] = er[t];
   }
   if (ch.src >= 0) renv_store_sym(rslot(aspill, ch.src), lane, eb);
 }
 
-// Per (instance, site): sum of its steps' real adjoints in step order, written
-// as a complex slot (imaginary parts zero) for k_site_bwd.
+// Per (instance, site): sum of its steps' real adjoints (radj_store's compact
+// layout, 512 >> tail-count entries) in step order, written as a complex slot
+// (imaginary parts zero, zeros beyond the pulled-back dimensions) for
+// k_site_bwd, which resumes the local program before the pulled-back halves.
 extern "C" __global__ void __launch_bounds__(256)
 k_rsite_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
                const double2 *__restrict__ aslot, double2 *__restrict__ abar) {
+  __shared__ double red[512];
   const InstDev inst = uload(insts + blockIdx.y);
   const int s = blockIdx.x;
   if (s >= inst.n) return;
   const ProbDev P = uload(probs + inst.prob);
   const int t = threadIdx.x;
+  const int tail = uload(P.site_tail + s), nt = tail & 3, k1 = (tail >> 4) & 15;
+  const bool r1 = k1 == LO_CTRL_R || k1 == LO_TARG_R;
+  const bool lh = nt == 2 || (nt == 1 && !r1), rh = nt == 2 || (nt == 1 && r1);
   const double2 *as = aslot + inst.step_off * GV_SLOT;
   double s0 = 0.0, s1 = 0.0;
-  const int q1 = uload(P.site_steps_begin + s + 1);
-  int q = uload(P.site_steps_begin + s);
-  for (; q + 8 <= q1; q += 8) {
-    double2 a[8];
+  if (2 * t < (512 >> nt)) {
+    const int q1 = uload(P.site_steps_begin + s + 1);
+    int q = uload(P.site_steps_begin + s);
+    for (; q + 8 <= q1; q += 8) {
+      double2 a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = as[(size_t)uload(P.site_steps + q + u) * GV_SLOT + t];  // doubles 2t, 2t+1
+      for (int u = 0; u < 8; ++u) a[u] = as[(size_t)uload(P.site_steps + q + u) * GV_SLOT + t];  // doubles 2t, 2t+1
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      s0 += a[u].x;
-      s1 += a[u].y;
+      for (int u = 0; u < 8; ++u) {
+        s0 += a[u].x;
+        s1 += a[u].y;
+      }
+    }
+    for (; q < q1; ++q) {
+      const double2 a = as[(size_t)uload(P.site_steps + q) * GV_SLOT + t];
+      s0 += a.x;
+      s1 += a.y;
     }
   }
-  for (; q < q1; ++q) {
-    const double2 a = as[(size_t)uload(P.site_steps + q) * GV_SLOT + t];
-    s0 += a.x;
-    s1 += a.y;
-  }
+  red[2 * t] = s0;
+  red[2 * t + 1] = s1;
+  __syncthreads();
   double2 *dst = abar + inst.site_off + (size_t)s * GV_SLOT;
-  dst[2 * t] = make_double2(s0, 0.0);
-  dst[2 * t + 1] = make_double2(s1, 0.0);
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = 2 * t + u, l = e >> 5, i = (e >> 4) & 1, r = e & 15;
+    const bool in = (!lh || l < 8) && (!rh || r < 8);
+    dst[e] = make_double2(in ? red[(l * 2 + i) * (rh ? 8 : 16) + r] : 0.0, 0.0);
+  }
 }
 
 // Real parts of the prepared site arena (imaginary parts are exactly zero on

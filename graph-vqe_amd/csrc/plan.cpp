@@ -652,6 +652,23 @@ void build_local_programs(Problem &P, const gvmps_op *ops, int n_ops) {
     P.lops.clear();
     P.lop_begin.assign(n + 1, 0);
   }
+  // Trailing CNOT halves of each program, at most one per bond side (each
+  // halves one bond of the site adjoint when pulled back).
+  P.site_tail.assign(n, 0);
+  for (int s = 0; s < n; ++s) {
+    int cnt = 0, code = 0;
+    bool left = false, right = false;
+    for (int o = P.lop_begin[s + 1] - 1; o >= P.lop_begin[s] && cnt < 2; --o) {
+      const int k = P.lops[o].kind;
+      const bool isr = k == LO_CTRL_R || k == LO_TARG_R, isl = k == LO_CTRL_L || k == LO_TARG_L;
+      if (!(isr || isl) || (isr && right) || (isl && left)) break;
+      right = right || isr;
+      left = left || isl;
+      code |= k << (4 + 4 * cnt);
+      ++cnt;
+    }
+    P.site_tail[s] = cnt | code;
+  }
   std::vector<std::vector<int>> by_param(P.nparams);
   for (int r = 0; r < (int)P.rot_param.size(); ++r)
     if (P.rot_param[r] >= 0) by_param[P.rot_param[r]].push_back(r);

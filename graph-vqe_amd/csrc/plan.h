@@ -59,6 +59,7 @@ struct Problem {
   std::vector<int> seed_begin;            // nspill+1
   std::vector<SeedTerm> seed_terms;
   std::vector<int> site_steps_begin, site_steps;
+  std::vector<int> site_tail;  // per site: trailing CNOT halves pulled back in the backward chains
   double flops_grad = 0;
   double flops_exec = 0, flops_nodes = 0;
 };
