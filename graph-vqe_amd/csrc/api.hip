@@ -598,7 +598,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.mark(4, stream);
     hip_check(kl_rsite_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_rsite_reduce launch");
     B.mark(5, stream);
-    hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad, 1),
+    hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad),
               "k_site_bwd launch");
     hip_check(kl_param_grad(ninst, B.max_params, stream, B.d_inst, B.d_probs, B.rgrad, B.grads),
               "k_param_grad launch");
@@ -631,7 +631,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   B.mark(4, stream);
   hip_check(kl_site_reduce(ninst, B.max_n, stream, B.d_inst, B.d_probs, B.aslot, B.abar), "k_site_reduce launch");
   B.mark(5, stream);
-  hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad, 0),
+  hip_check(kl_site_bwd(ninst, B.max_n, stream, B.d_inst, B.d_probs, params, B.abar, B.rotin, B.rgrad),
             "k_site_bwd launch");
   hip_check(kl_param_grad(ninst, B.max_params, stream, B.d_inst, B.d_probs, B.rgrad, B.grads),
             "k_param_grad launch");

@@ -153,8 +153,7 @@ hipError_t kl_chains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, 
 hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const double2 *aslot, double2 *abar);
 hipError_t kl_site_bwd(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad,
-                       int pulled_tail);
+                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad);
 hipError_t kl_param_grad(int ninst, int max_params, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                          const double *rgrad, double *grads);
 // kernel launchers (kernels_sample.hip)

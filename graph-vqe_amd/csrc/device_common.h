@@ -242,3 +242,90 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// ------------------------------------------- site adjoints of the backward
+// A step's site adjoint arrives in the D layout: lane a16 + 16 g4, register
+// v holds Mbar_i[kdx(g4, v)][a16] (rows = left bond, columns = right bond) in
+// m0 (i = 0) and m1 (i = 1).  Before it is stored it is pulled back through
+// the CNOT halves that end its site's local program (ProbDev::site_tail:
+// count | last kind << 4 | kind before it << 8, at most one per bond side),
+// the sums the local backward would otherwise make per site:
+//   control, right:  Abar[l,i,r] = A'bar[l,i,2r+i]
+//   target,  right:  Abar[l,i,r] = A'bar[l,i,2r] + A'bar[l,i^1,2r+1]
+//   control, left:   Abar[l,i,r] = A'bar[2l+i,i,r]
+//   target,  left:   Abar[l,i,r] = A'bar[2l,i,r] + A'bar[2l+1,i^1,r]
+// (a CNOT half appends the new bond bit as the index's lowest bit).  Row
+// parity is g4 & 1 and column parity a16 & 1, so the first pull-back leaves
+// each lane one component i (its column parity after a right half, its row
+// parity after a left half), pairing lanes 1 or 16 apart; a second one, on
+// the other bond, pairs lanes 17 apart and keeps half of the lanes.  The
+// result is stored compactly, ((l * 2 + i) * rc + r) over the pulled-back
+// dimensions: 16x2x16 (no pull-back), 16x2x8, 8x2x16 or 8x2x8.
+
+__device__ __forceinline__ bool tail_right_first(int tail) {
+  const int k1 = (tail >> 4) & 15;
+  return k1 == gv::LO_CTRL_R || k1 == gv::LO_TARG_R;
+}
+// Halved dimensions after the pull-back: rows (left bond), columns (right bond).
+__device__ __forceinline__ bool tail_rows_half(int tail) {
+  const int nt = tail & 3;
+  return nt == 2 || (nt == 1 && !tail_right_first(tail));
+}
+__device__ __forceinline__ bool tail_cols_half(int tail) {
+  const int nt = tail & 3;
+  return nt == 2 || (nt == 1 && tail_right_first(tail));
+}
+
+// Where this lane's pulled-back values go: register v lands at
+// ((lh ? row >> 1 : row) * 2 + c) * rc + col, row = kdx(g4, v), if keep.
+struct TailMap {
+  bool keep, lh;
+  int c, rc, col;
+};
+__device__ __forceinline__ int tail_index(const TailMap &m, int g4, int v) {
+  const int row = kdx(g4, v);
+  return ((m.lh ? row >> 1 : row) * 2 + m.c) * m.rc + m.col;
+}
+
+// The pulled-back values of this lane per register, and where they go
+// (tail count >= 1).  y may alias neither m0 nor m1.
+__device__ __forceinline__ TailMap tail_pull(int tail, int a16, int g4, const d4 &m0, const d4 &m1, double y[4]) {
+  const int nt = tail & 3, k1 = (tail >> 4) & 15;
+  const bool r1 = k1 == gv::LO_CTRL_R || k1 == gv::LO_TARG_R;
+  TailMap m;
+  m.c = r1 ? (a16 & 1) : (g4 & 1);
+  if (k1 == gv::LO_TARG_R || k1 == gv::LO_TARG_L) {
+    const int x = r1 ? 1 : 16;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = m0[v] + __shfl_xor(m1[v], x);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = m.c ? m1[v] : m0[v];
+  }
+  m.keep = true;
+  if (nt == 2) {
+    const int k2 = (tail >> 8) & 15;
+    if (k2 == gv::LO_TARG_R || k2 == gv::LO_TARG_L) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) y[v] += __shfl_xor(y[v], 17);
+      m.keep = ((r1 ? g4 : a16) & 1) == 0;
+    } else {
+      m.keep = (a16 & 1) == (g4 & 1);
+    }
+  }
+  m.lh = !r1 || nt == 2;
+  const bool rh = r1 || nt == 2;
+  m.rc = rh ? 8 : 16;
+  m.col = rh ? a16 >> 1 : a16;
+  return m;
+}
+
+// Compact index of site-slot element e = (l*2 + i)*16 + r, or -1 when the
+// element lies beyond the pulled-back dimensions (its adjoint is zero there).
+__device__ __forceinline__ int tail_compact_index(int tail, int e) {
+  const int l = e >> 5, i = (e >> 4) & 1, r = e & 15;
+  const bool lh = tail_rows_half(tail), rh = tail_cols_half(tail);
+  if ((lh && l >= 8) || (rh && r >= 8)) return -1;
+  return (l * 2 + i) * (rh ? 8 : 16) + r;
+}
+

@@ -427,7 +427,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       cacc_get(ea, er, ei);
     }
     // Mbar_i = 2 E G_i: G_i to D layout through LDS
-    double2 *slot = as + (size_t)step * GV_SLOT;
+    d4 mbr[2], mbi[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const d4 &gr = i ? g1r : g0r;
@@ -460,8 +460,28 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
         else cmm(E.re, E.im, dr, di, false, false, ma);
         cacc_get(ma, mr, mi);
       }
+      mbr[i] = mr;
+      mbi[i] = mi;
+    }
+    // pulled back through the site's trailing CNOT halves (tail_pull), compact
+    {
+      double2 *slot = as + (size_t)step * GV_SLOT;
+      const int tail = uload(P.site_tail + sd.site);
+      if ((tail & 3) == 0) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) slot[sidx(kdx(g4, v), i, a16)] = make_double2(mr[v], mi[v]);
+        for (int v = 0; v < 4; ++v) {
+          slot[sidx(kdx(g4, v), 0, a16)] = make_double2(mbr[0][v], mbi[0][v]);
+          slot[sidx(kdx(g4, v), 1, a16)] = make_double2(mbr[1][v], mbi[1][v]);
+        }
+      } else {
+        double yr[4], yi[4];
+        const TailMap tm = tail_pull(tail, a16, g4, mbr[0], mbr[1], yr);
+        tail_pull(tail, a16, g4, mbi[0], mbi[1], yi);
+        if (tm.keep) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) slot[tail_index(tm, g4, v)] = make_double2(yr[v], yi[v]);
+        }
+      }
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -473,56 +493,66 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
 }
 
 // ------------------------------------------------------- site adjoints
-// grid (max sites, instances), 256 threads: thread t owns doubles 4t..4t+3
-// of the 16x2x16 complex slot.
+// grid (max sites, instances), 256 threads: thread t sums complex elements
+// 2t, 2t+1 of the compact pulled-back adjoints (512 >> tail count elements,
+// tail_pull) over the site's steps in step order, then the block
+// writes the 16x2x16 slot, zero beyond the pulled-back dimensions.
 extern "C" __global__ void __launch_bounds__(256)
 k_site_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
               const double2 *__restrict__ aslot, double2 *__restrict__ abar) {
+  __shared__ double2 red[512];
   const InstDev inst = uload(insts + blockIdx.y);
   const int s = blockIdx.x;
   if (s >= inst.n) return;
   const ProbDev P = uload(probs + inst.prob);
   const int t = threadIdx.x;
+  const int tail = uload(P.site_tail + s);
   const double2 *as = aslot + inst.step_off * GV_SLOT;
   double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
-  const int q1 = uload(P.site_steps_begin + s + 1);
-  int q = uload(P.site_steps_begin + s);
-  // eight slots in flight per thread; the additions keep step order
-  for (; q + 8 <= q1; q += 8) {
-    double2 a[8], c[8];
+  if (2 * t < (512 >> (tail & 3))) {
+    const int q1 = uload(P.site_steps_begin + s + 1);
+    int q = uload(P.site_steps_begin + s);
+    // eight slots in flight per thread; the additions keep step order
+    for (; q + 8 <= q1; q += 8) {
+      double2 a[8], c[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const double2 *src = as + (size_t)uload(P.site_steps + q + u) * GV_SLOT;
-      a[u] = src[2 * t];
-      c[u] = src[2 * t + 1];
+      for (int u = 0; u < 8; ++u) {
+        const double2 *src = as + (size_t)uload(P.site_steps + q + u) * GV_SLOT;
+        a[u] = src[2 * t];
+        c[u] = src[2 * t + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        s0 = cadd(s0, a[u]);
+        s1 = cadd(s1, c[u]);
+      }
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      s0 = cadd(s0, a[u]);
-      s1 = cadd(s1, c[u]);
+    for (; q < q1; ++q) {
+      const double2 *src = as + (size_t)uload(P.site_steps + q) * GV_SLOT;
+      s0 = cadd(s0, src[2 * t]);
+      s1 = cadd(s1, src[2 * t + 1]);
     }
   }
-  for (; q < q1; ++q) {
-    const double2 *src = as + (size_t)uload(P.site_steps + q) * GV_SLOT;
-    const double2 a = src[2 * t], c = src[2 * t + 1];
-    s0 = cadd(s0, a);
-    s1 = cadd(s1, c);
-  }
+  red[2 * t] = s0;
+  red[2 * t + 1] = s1;
+  __syncthreads();
   double2 *dst = abar + inst.site_off + (size_t)s * GV_SLOT;
-  dst[2 * t] = s0;
-  dst[2 * t + 1] = s1;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int k = tail_compact_index(tail, 2 * t + u);
+    dst[2 * t + u] = k >= 0 ? red[k] : make_double2(0.0, 0.0);
+  }
 }
 
 // Local program in reverse.  Rotation k: T_{k+1} = U T_k, so
 // dE/dtheta = Re sum conj(Tbar_{k+1}) (dU/dtheta T_k) and Tbar_k = U^H Tbar_{k+1};
-// CNOT halves pull the adjoint back through their index maps.  With
-// pulled_tail != 0 the adjoint in abar has already been pulled back through
-// the site's trailing CNOT halves (ProbDev::site_tail), and the program
-// resumes before them.
+// CNOT halves pull the adjoint back through their index maps.  The adjoint
+// in abar has already been pulled back through the site's trailing CNOT
+// halves (ProbDev::site_tail) by the backward chains, so the program resumes
+// before them.
 extern "C" __global__ void __launch_bounds__(64)
 k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
-           const double2 *__restrict__ abar, const double2 *__restrict__ rotin, double *__restrict__ rgrad,
-           int pulled_tail) {
+           const double2 *__restrict__ abar, const double2 *__restrict__ rotin, double *__restrict__ rgrad) {
   __shared__ double2 buf[2][GV_SLOT];
   const int lane = threadIdx.x;
   const InstDev inst = insts[blockIdx.y];
@@ -535,7 +565,7 @@ k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
   for (int k = 0; k < 8; ++k) buf[0][lane + 64 * k] = src[lane + 64 * k];
   __syncthreads();
   int cur = 0;
-  const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1] - (pulled_tail ? (P.site_tail[s] & 3) : 0);
+  const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1] - (P.site_tail[s] & 3);
   for (int o = o1 - 1; o >= o0; --o) {
     const LocalOp op = P.lops[o];
     double2 U[4], dU[4];
@@ -633,10 +663,8 @@ hipError_t kl_site_reduce(int ninst, int max_n, hipStream_t s, const InstDev *in
 }
 
 hipError_t kl_site_bwd(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad,
-                       int pulled_tail) {
-  hipLaunchKernelGGL(k_site_bwd, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, abar, rotin, rgrad,
-                     pulled_tail);
+                       const double *params, const double2 *abar, const double2 *rotin, double *rgrad) {
+  hipLaunchKernelGGL(k_site_bwd, dim3(max_n, ninst), dim3(64), 0, s, insts, probs, params, abar, rotin, rgrad);
   return hipGetLastError();
 }
 

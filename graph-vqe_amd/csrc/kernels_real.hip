@@ -274,23 +274,11 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
   for (int v = 0; v < 4; ++v) e.v[v] = kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
 }
 
-// Stores one step's site adjoint, given as rows kdx(g4, v), column a16 of
-// Mbar_0 (m0) and Mbar_1 (m1), pulled back through the CNOT halves that end
-// the site's local program (`tail`, see ProbDev::site_tail), in the compact
-// layout ((l * 2 + i) * rc + r) of the pulled-back dimensions: 16x2x16,
-// 16x2x8, 8x2x16 or 8x2x8.  The pull-backs (new bond bit = the index's lowest
-// bit, as in k_site_bwd):
-//   control, right:  Abar[l,i,r] = A'bar[l,i,2r+i]
-//   target,  right:  Abar[l,i,r] = A'bar[l,i,2r] + A'bar[l,i^1,2r+1]
-// and the same on rows for the left halves.  Row parity is g4 & 1 and column
-// parity a16 & 1, so the first pull-back leaves each lane one component i
-// (its column parity after a right half, its row parity after a left half),
-// pairing lanes 1 or 16 apart; a second one, on the other bond, pairs lanes
-// 17 apart and keeps half of the lanes.  The sums are the ones k_site_bwd
-// makes, per step instead of per site.
+// Stores one step's site adjoint (m0, m1: Mbar_0, Mbar_1 in the D layout),
+// pulled back through the CNOT halves that end its site's program, in the
+// compact layout of tail_pull (device_common.h).
 __device__ __forceinline__ void radj_store(double *slot, int tail, int a16, int g4, const d4 &m0, const d4 &m1) {
-  const int nt = tail & 3;
-  if (nt == 0) {
+  if ((tail & 3) == 0) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       slot[sidx(kdx(g4, v), 0, a16)] = m0[v];
@@ -298,37 +286,11 @@ __device__ __forceinline__ void radj_store(double *slot, int tail, int a16, int 
     }
     return;
   }
-  const int k1 = (tail >> 4) & 15;
-  const bool r1 = k1 == LO_CTRL_R || k1 == LO_TARG_R;
-  const int c = r1 ? (a16 & 1) : (g4 & 1);
   double y[4];
-  if (k1 == LO_TARG_R || k1 == LO_TARG_L) {
-    const int x = r1 ? 1 : 16;
+  const TailMap tm = tail_pull(tail, a16, g4, m0, m1, y);
+  if (tm.keep) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) y[v] = m0[v] + __shfl_xor(m1[v], x);
-  } else {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) y[v] = c ? m1[v] : m0[v];
-  }
-  bool keep = true;
-  if (nt == 2) {
-    const int k2 = (tail >> 8) & 15;
-    if (k2 == LO_TARG_R || k2 == LO_TARG_L) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) y[v] += __shfl_xor(y[v], 17);
-      keep = ((r1 ? g4 : a16) & 1) == 0;
-    } else {
-      keep = (a16 & 1) == (g4 & 1);
-    }
-  }
-  const bool lh = !r1 || nt == 2, rh = r1 || nt == 2;
-  const int rc = rh ? 8 : 16, col = rh ? a16 >> 1 : a16;
-  if (keep) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = kdx(g4, v);
-      slot[((lh ? row >> 1 : row) * 2 + c) * rc + col] = y[v];
-    }
+    for (int v = 0; v < 4; ++v) slot[tail_index(tm, g4, v)] = y[v];
   }
 }
 
@@ -595,9 +557,7 @@ k_rsite_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pr
   if (s >= inst.n) return;
   const ProbDev P = uload(probs + inst.prob);
   const int t = threadIdx.x;
-  const int tail = uload(P.site_tail + s), nt = tail & 3, k1 = (tail >> 4) & 15;
-  const bool r1 = k1 == LO_CTRL_R || k1 == LO_TARG_R;
-  const bool lh = nt == 2 || (nt == 1 && !r1), rh = nt == 2 || (nt == 1 && r1);
+  const int tail = uload(P.site_tail + s), nt = tail & 3;
   const double2 *as = aslot + inst.step_off * GV_SLOT;
   double s0 = 0.0, s1 = 0.0;
   if (2 * t < (512 >> nt)) {
@@ -625,9 +585,8 @@ k_rsite_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pr
   double2 *dst = abar + inst.site_off + (size_t)s * GV_SLOT;
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int e = 2 * t + u, l = e >> 5, i = (e >> 4) & 1, r = e & 15;
-    const bool in = (!lh || l < 8) && (!rh || r < 8);
-    dst[e] = make_double2(in ? red[(l * 2 + i) * (rh ? 8 : 16) + r] : 0.0, 0.0);
+    const int k = tail_compact_index(tail, 2 * t + u);
+    dst[2 * t + u] = make_double2(k >= 0 ? red[k] : 0.0, 0.0);
   }
 }
 
