@@ -575,7 +575,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
-      hip_check(kl_rchains_gfwd(ni, B.wpb, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+      hip_check(kl_rchains_gfwd(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                 B.spills, B.parts, B.estore, B.gleafs, B.leafs),
                 "k_rchains_gfwd launch");
     }
@@ -589,7 +589,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
       const int ni = B.gn_items[g];
       if (ni == 0) continue;
       B.kbegin(stream);
-      hip_check(kl_rchains_bwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+      hip_check(kl_rchains_bwd(ni, B.wpb, B.iz, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                B.spills, B.adjspill,
                                B.estore, B.gleafs, B.aslot),
                 "k_rchains_bwd launch");

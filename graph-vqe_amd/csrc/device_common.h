@@ -38,6 +38,9 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 #define GV_CIZ_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
 #endif
 #define GV_FIVE_WAVES __attribute__((amdgpu_waves_per_eu(5, 5)))
+// The I/Z forms of the reverse-gradient chains (forward and backward).
+#define GV_FWD_IZ_WAVES __attribute__((amdgpu_waves_per_eu(5, 5)))
+#define GV_BWD_IZ_WAVES __attribute__((amdgpu_waves_per_eu(5, 5)))
 #define GV_IZ_TERM_UNROLL 1
 
 // Index of this wavefront in its workgroup as a wave-uniform value.  Without
@@ -138,6 +141,29 @@ __device__ __forceinline__ void env_store_cm(double2 *p, int lane, const Env &e)
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) p[kdx(g, t) * 16 + a] = make_double2(e.re[t], e.im[t]);
+}
+
+// Non-temporal forms, for slots written once and read once much later (the
+// reverse gradient's checkpointed step environments).
+__device__ __forceinline__ void nt_store2(double2 *p, double re, double im) {
+  __builtin_nontemporal_store(re, &p->x);
+  __builtin_nontemporal_store(im, &p->y);
+}
+
+__device__ __forceinline__ void env_load_cm_nt(const double2 *p, int lane, Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const double2 *q = p + kdx(g, t) * 16 + a;
+    e.re[t] = __builtin_nontemporal_load(&q->x);
+    e.im[t] = __builtin_nontemporal_load(&q->y);
+  }
+}
+
+__device__ __forceinline__ void env_store_cm_nt(double2 *p, int lane, const Env &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) nt_store2(p + kdx(g, t) * 16 + a, e.re[t], e.im[t]);
 }
 
 // Forest environments of Hermitian Pauli words are Hermitian, and so are

@@ -161,7 +161,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     Frag f;
     frag_load(sites + (size_t)sd.site * GV_SLOT, lane, f, false);
     const int cs = uload(P.ck_slot + step);
-    if (cs >= 0) env_store_cm(es + (size_t)cs * GV_ENV, lane, e);
+    if (cs >= 0) env_store_cm_nt(es + (size_t)cs * GV_ENV, lane, e);
     d4 x0r, x0i, x1r, x1i;
     prod_x(e, f, 0, x0r, x0i);
     prod_x(e, f, 1, x1r, x1i);
@@ -341,11 +341,11 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
     Env E;
     const int cs = uload(P.ck_slot + step);
     if (cs >= 0) {
-      env_load_cm(es + (size_t)cs * GV_ENV, lane, E);
+      env_load_cm_nt(es + (size_t)cs * GV_ENV, lane, E);
     } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
       int s0 = st;
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
-      env_load_cm(es + (size_t)uload(P.ck_slot + ch.step0 + s0) * GV_ENV, lane, E);
+      env_load_cm_nt(es + (size_t)uload(P.ck_slot + ch.step0 + s0) * GV_ENV, lane, E);
       for (int q = s0; q < st; ++q) advance(P, sites, ch.step0 + q, lane, E);
     }
     Frag fa;  // E layout of M_i (A operand form)
@@ -463,15 +463,16 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       mbr[i] = mr;
       mbi[i] = mi;
     }
-    // pulled back through the site's trailing CNOT halves (tail_pull), compact
+    // pulled back through the site's trailing CNOT halves (tail_pull), compact;
+    // non-temporal, like the checkpoints: read once, by the site reduction
     {
       double2 *slot = as + (size_t)step * GV_SLOT;
       const int tail = uload(P.site_tail + sd.site);
       if ((tail & 3) == 0) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          slot[sidx(kdx(g4, v), 0, a16)] = make_double2(mbr[0][v], mbi[0][v]);
-          slot[sidx(kdx(g4, v), 1, a16)] = make_double2(mbr[1][v], mbi[1][v]);
+          nt_store2(slot + sidx(kdx(g4, v), 0, a16), mbr[0][v], mbi[0][v]);
+          nt_store2(slot + sidx(kdx(g4, v), 1, a16), mbr[1][v], mbi[1][v]);
         }
       } else {
         double yr[4], yi[4];
@@ -479,7 +480,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
         tail_pull(tail, a16, g4, mbi[0], mbi[1], yi);
         if (tm.keep) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) slot[tail_index(tm, g4, v)] = make_double2(yr[v], yi[v]);
+          for (int v = 0; v < 4; ++v) nt_store2(slot + tail_index(tm, g4, v), yr[v], yi[v]);
         }
       }
     }

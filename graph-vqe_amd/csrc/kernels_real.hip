@@ -55,17 +55,19 @@ __device__ __forceinline__ void renv_store(double *p, int lane, const REnv &e) {
 // Column-major slots (element (a, k) at k * 16 + a): each store or load
 // instruction covers four whole 128-byte rows, where the row-major form
 // touches sixteen.  Used for slots that only these kernels write and read
-// (the gradient's checkpointed step environments).
+// (the gradient's checkpointed step environments).  Each is written once by
+// the forward pass and read once by the backward pass, long after, so both
+// go non-temporal instead of displacing the hand-off slots in L2.
 __device__ __forceinline__ void renv_load_cm(const double *p, int lane, REnv &e) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) e.v[t] = p[kdx(g, t) * 16 + a];
+  for (int t = 0; t < 4; ++t) e.v[t] = __builtin_nontemporal_load(p + kdx(g, t) * 16 + a);
 }
 
 __device__ __forceinline__ void renv_store_cm(double *p, int lane, const REnv &e) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) p[kdx(g, t) * 16 + a] = e.v[t];
+  for (int t = 0; t < 4; ++t) __builtin_nontemporal_store(e.v[t], p + kdx(g, t) * 16 + a);
 }
 
 __device__ __forceinline__ void renv_unit(int lane, REnv &e) {
@@ -276,13 +278,14 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
 
 // Stores one step's site adjoint (m0, m1: Mbar_0, Mbar_1 in the D layout),
 // pulled back through the CNOT halves that end its site's program, in the
-// compact layout of tail_pull (device_common.h).
+// compact layout of tail_pull (device_common.h).  Non-temporal: the site
+// reduction reads it once, after the whole backward pass.
 __device__ __forceinline__ void radj_store(double *slot, int tail, int a16, int g4, const d4 &m0, const d4 &m1) {
   if ((tail & 3) == 0) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      slot[sidx(kdx(g4, v), 0, a16)] = m0[v];
-      slot[sidx(kdx(g4, v), 1, a16)] = m1[v];
+      __builtin_nontemporal_store(m0[v], slot + sidx(kdx(g4, v), 0, a16));
+      __builtin_nontemporal_store(m1[v], slot + sidx(kdx(g4, v), 1, a16));
     }
     return;
   }
@@ -290,7 +293,7 @@ __device__ __forceinline__ void radj_store(double *slot, int tail, int a16, int 
   const TailMap tm = tail_pull(tail, a16, g4, m0, m1, y);
   if (tm.keep) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) slot[tail_index(tm, g4, v)] = y[v];
+    for (int v = 0; v < 4; ++v) __builtin_nontemporal_store(y[v], slot + tail_index(tm, g4, v));
   }
 }
 
@@ -314,11 +317,14 @@ __device__ __forceinline__ void radj_store(double *slot, int tail, int a16, int 
 // forest): deferred term ends go to leaf slots for k_rleaf_energy, exactly as
 // in the energy pass.  The backward pass reads only what this pass stores, so
 // it keeps its own generation order.
-extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
-k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
-               const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
-               double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
+// IZ: every step has I/Z children only (one pass, as in rchains_body).
+template <bool IZ>
+__device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int nitems,
+                                           const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                           const double *__restrict__ rgarena, const double *__restrict__ rgarenaT,
+                                           double2 *__restrict__ spills, double *__restrict__ parts,
+                                           double2 *__restrict__ estore, double2 *__restrict__ gleafs,
+                                           double2 *__restrict__ leafs) {
   const int lane = threadIdx.x & 63;
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
@@ -350,13 +356,13 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
     REnv cont;
 #pragma nounroll
-    for (int pass = 0; pass < 2; ++pass) {
-      if (!(sd.need & (1 << pass))) continue;
+    for (int pass = 0; pass < (IZ ? 1 : 2); ++pass) {
+      if (!IZ && !(sd.need & (1 << pass))) continue;
       const d4 ya = rprod_y(pass ? x1 : x0, f, 0);
       const d4 yb = rprod_y(pass ? x0 : x1, f, 1);
       for (int c = 0; c < sd.nchild; ++c) {
         const ChildDev cd = uload(P.children + sd.child0 + c);
-        if ((cd.kind == CK_X) != (pass == 1)) continue;
+        if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
         REnv ce;
 #pragma unroll
         for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
@@ -390,15 +396,33 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
   if (lane == 0) parts[inst.part_off + it.y] = acc;
 }
 
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
+k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+               const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
+               double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
+  rgfwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_FWD_IZ_WAVES
+k_rchains_gfwd_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                  const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+                  const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
+                  double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
+  rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs);
+}
+
 // Backward (k_chains_bwd in real arithmetic).  For a step with input E and
 // site matrices M_i: Ybar00 = I+Z, Ybar11 = I-Z, Ybar01 = Ybar10 = X (child
 // adjoints), G_i = sum_j M_j Ybar_ji, Ebar = sum_i G_i M_i^T, Mbar_i = 2 E G_i.
-extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
-k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-              const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
-              const double *__restrict__ rgarenaT, const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
-              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
-  __shared__ double tr[4][16][17];
+// IZ: I/Z children only, so Ybar01 = Ybar10 = 0 and G_i = Ybar_ii M_i.
+template <bool IZ>
+__device__ __forceinline__ void rbwd_body(const int2 *__restrict__ items, int nitems,
+                                          const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                          const double *__restrict__ rgarena, const double *__restrict__ rgarenaT,
+                                          const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
+                                          const double2 *__restrict__ estore, const double2 *__restrict__ gleafs,
+                                          double2 *__restrict__ aslot, double (*tr)[16][17]) {
   const int lane = threadIdx.x & 63;
   const int wv = wave_id();
   const int nb = gridDim.x, b = blockIdx.x;
@@ -441,7 +465,7 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     rfrag_load(sites_a + (size_t)sd.site * GV_SLOT, lane, fa);
     REnv y00, y11, y01;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) y00.v[t] = y11.v[t] = y01.v[t] = 0.0;
+    for (int t = 0; t < 4; ++t) y00.v[t] = y11.v[t] = y01.v[t] = 0.0;  // y01 stays 0 when IZ
     for (int c = 0; c < sd.nchild; ++c) {
       const ChildDev cd = uload(P.children + sd.child0 + c);
       REnv cb;
@@ -479,20 +503,20 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         if (cd.kind == CK_I) { y00.v[t] += cb.v[t]; y11.v[t] += cb.v[t]; }
-        else if (cd.kind == CK_Z) { y00.v[t] += cb.v[t]; y11.v[t] -= cb.v[t]; }
+        else if (IZ || cd.kind == CK_Z) { y00.v[t] += cb.v[t]; y11.v[t] -= cb.v[t]; }
         else y01.v[t] += cb.v[t];  // X: Ybar01 = Ybar10
       }
     }
     // G_i (as the next products' A operand): G_0 = Ybar00 M_0 + Ybar01 M_1, G_1 = Ybar11 M_1 + Ybar10 M_0
     d4 g0 = {0.0, 0.0, 0.0, 0.0}, g1 = {0.0, 0.0, 0.0, 0.0};
-    if (sd.need & 1) {
+    if (IZ || (sd.need & 1)) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         g0 = mfma(y00.v[t], fa.v[0][t], g0);
         g1 = mfma(y11.v[t], fa.v[1][t], g1);
       }
     }
-    if (sd.need & 2) {
+    if (!IZ && (sd.need & 2)) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         g0 = mfma(y01.v[t], fa.v[1][t], g0);
@@ -542,6 +566,25 @@ k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
     for (int t = 0; t < 4; ++t) eb.v[t] = er[t];
   }
   if (ch.src >= 0) renv_store_sym(rslot(aspill, ch.src), lane, eb);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
+k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+              const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+              const double *__restrict__ rgarenaT, const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
+              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
+  __shared__ double tr[4][16][17];
+  rbwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, adjspill, estore, gleafs, aslot, tr);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_BWD_IZ_WAVES
+k_rchains_bwd_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                 const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+                 const double *__restrict__ rgarenaT, const double2 *__restrict__ spills,
+                 double2 *__restrict__ adjspill, const double2 *__restrict__ estore,
+                 const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
+  __shared__ double tr[4][16][17];
+  rbwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, adjspill, estore, gleafs, aslot, tr);
 }
 
 // Per (instance, site): sum of its steps' real adjoints (radj_store's compact
@@ -667,18 +710,18 @@ hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *i
   return hipGetLastError();
 }
 
-hipError_t kl_rchains_gfwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_rchains_gfwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
                            double2 *estore, double2 *gleafs, double2 *leafs) {
-  hipLaunchKernelGGL(k_rchains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
+  hipLaunchKernelGGL(iz ? k_rchains_gfwd_iz : k_rchains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
                      rgarena, rgarena + nsite, spills, parts, estore, gleafs, leafs);
   return hipGetLastError();
 }
 
-hipError_t kl_rchains_bwd(int nitems, int wpb, hipStream_t s, const int2 *items, const InstDev *insts,
+hipError_t kl_rchains_bwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                           const ProbDev *probs, const double *rgarena, long long nsite, const double2 *spills,
                           double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot) {
-  hipLaunchKernelGGL(k_rchains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
+  hipLaunchKernelGGL(iz ? k_rchains_bwd_iz : k_rchains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
                      rgarena, rgarena + nsite, spills, adjspill, estore, gleafs, aslot);
   return hipGetLastError();
 }
