@@ -59,6 +59,12 @@ struct Batch {
   bool iz = false;    // every problem's forest has I/Z children only: k_rchains*_iz
   int arith = 0;              // 0: real path when `real`, 1: complex path always
   double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
+  // persistent forest launch (order 2): phase table, barrier and pop words
+  PhaseDev *d_phases = nullptr;
+  int nphase = 0;
+  int *psync = nullptr;   // 4 barrier words + 8 pop counters per phase, zero between launches
+  bool phases_dirty = true;
+  int persist_grid[2] = {-1, -1};  // resident workgroups of k_rforest_persist / _iz
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   const int2 *ggen_items(int g) const { return order ? gitems_len[g] : gitems[g]; }
   // captured evaluate + sample sequence
@@ -133,6 +139,11 @@ hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *i
                       double2 *terms, double *parts);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
+int persist_grid(bool iz, int device);
+hipError_t kl_rforest_persist(int grid, bool iz, hipStream_t s, const PhaseDev *phases, int nphase, int *sync,
+                              const InstDev *insts, int ninst, const ProbDev *probs, const double *rarena,
+                              long long nsite, double2 *leafs, double2 *spills, double *parts, double *energies,
+                              int *status);
 hipError_t kl_rchains_gfwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
                            double2 *estore, double2 *gleafs, double2 *leafs);

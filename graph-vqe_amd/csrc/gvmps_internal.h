@@ -185,6 +185,18 @@ struct InstDev {
 };
 
 // Status codes written by kernels (per instance).
-enum Status : int { ST_OK = 0, ST_GUARD = 2, ST_NONFINITE = 1 };
+enum Status : int { ST_OK = 0, ST_GUARD = 2, ST_NONFINITE = 1, ST_SCHED = 3 };
+
+// One phase of the persistent forest launch (k_rforest_persist): a forest
+// generation's chains, the deferred leaves, or the energy reduction.  Chain
+// and leaf items are stored as for the generation launches, each XCD's share
+// of them contiguous: share[x] .. share[x + 1].
+struct PhaseDev {
+  const int* items;    // (instance, item) pairs
+  int kind;            // 0 chains, 1 deferred leaves, 2 energy reduction (a workgroup per instance)
+  int count;
+  int share[9];
+  int pad;
+};
 
 }  // namespace gv

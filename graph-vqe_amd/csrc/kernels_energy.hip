@@ -312,33 +312,8 @@ extern "C" __global__ void __launch_bounds__(256)
 k_reduce_energy(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
                 const double *__restrict__ parts, double *__restrict__ energies, int with_leaves) {
   __shared__ double ssum[256], scomp[256];
-  const int tid = threadIdx.x;
   const InstDev inst = insts[blockIdx.x];
-  const ProbDev P = probs[inst.prob];
-  double sum = 0.0, comp = 0.0;
-  auto add = [&](double term) {
-    const double t = sum + term;
-    comp += (fabs(sum) >= fabs(term)) ? (sum - t) + term : (term - t) + sum;
-    sum = t;
-  };
-  for (int k = tid; k < P.nterms; k += 256)
-    if (!P.active[k]) add(P.coeffs[2 * k]);
-  const int nparts = P.nchains + (with_leaves ? P.nleaf : 0);
-  for (int c = tid; c < nparts; c += 256) add(parts[inst.part_off + c]);
-  ssum[tid] = sum;
-  scomp[tid] = comp;
-  __syncthreads();
-  for (int off = 128; off >= 1; off >>= 1) {
-    if (tid < off) {
-      const double a = ssum[tid], b = ssum[tid + off];
-      const double t = a + b;
-      const double e = (fabs(a) >= fabs(b)) ? (a - t) + b : (b - t) + a;
-      ssum[tid] = t;
-      scomp[tid] += scomp[tid + off] + e;
-    }
-    __syncthreads();
-  }
-  if (tid == 0) energies[inst.out] = ssum[0] + scomp[0];
+  reduce_energy_block(inst, probs[inst.prob], parts, energies, with_leaves, ssum, scomp);
 }
 
 // ------------------------------------------------------------ host launchers

@@ -131,18 +131,11 @@ __device__ __forceinline__ d4 rprod_y(const d4 &x, const RFrag &f, int j) {
 // with fewer live registers the kernel runs more waves per SIMD.  The
 // arithmetic per instance is the same in both forms.
 template <bool TERMS, bool IZ>
-__device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int nitems,
-                                             const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-                                             const double *__restrict__ rarena, const double *__restrict__ rarenaT,
-                                             double2 *__restrict__ leafs, double2 *__restrict__ spills,
-                                             double2 *__restrict__ terms, double *__restrict__ parts) {
-  const int lane = threadIdx.x & 63;
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
-  const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
-  const int w = logical * (int)(blockDim.x >> 6) + wave_id();
-  if (w >= nitems) return;
-  const int2 it = uload(items + w);
+__device__ __forceinline__ void rchain_item(const int2 it, const int lane, const InstDev *__restrict__ insts,
+                                            const ProbDev *__restrict__ probs, const double *__restrict__ rarena,
+                                            const double *__restrict__ rarenaT, double2 *__restrict__ leafs,
+                                            double2 *__restrict__ spills, double2 *__restrict__ terms,
+                                            double *__restrict__ parts) {
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
@@ -216,15 +209,23 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
   }
 }
 
-template <bool TERMS>
-__device__ __forceinline__ void rleaf_body(const int2 *__restrict__ items, int nitems,
-                                           const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-                                           const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
-                                           double2 *__restrict__ terms, double *__restrict__ parts) {
-  const int lane = threadIdx.x & 63;
+template <bool TERMS, bool IZ>
+__device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int nitems,
+                                             const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                             const double *__restrict__ rarena, const double *__restrict__ rarenaT,
+                                             double2 *__restrict__ leafs, double2 *__restrict__ spills,
+                                             double2 *__restrict__ terms, double *__restrict__ parts) {
   const int w = xcd_item();
   if (w >= nitems) return;
-  const int2 it = uload(items + w);
+  rchain_item<TERMS, IZ>(uload(items + w), threadIdx.x & 63, insts, probs, rarena, rarenaT, leafs, spills, terms,
+                         parts);
+}
+
+template <bool TERMS>
+__device__ __forceinline__ void rleaf_item(const int2 it, const int lane, const InstDev *__restrict__ insts,
+                                           const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs,
+                                           const double2 *__restrict__ spills, double2 *__restrict__ terms,
+                                           double *__restrict__ parts) {
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const LeafDev lf = uload(P.leaves + it.y);
@@ -251,6 +252,90 @@ __device__ __forceinline__ void rleaf_body(const int2 *__restrict__ items, int n
   if (!TERMS) {
     acc = wave_sum(acc);
     if (lane == 0) parts[inst.part_off + P.nchains + it.y] = acc;
+  }
+}
+
+template <bool TERMS>
+__device__ __forceinline__ void rleaf_body(const int2 *__restrict__ items, int nitems,
+                                           const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+                                           const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
+                                           double2 *__restrict__ terms, double *__restrict__ parts) {
+  const int w = xcd_item();
+  if (w >= nitems) return;
+  rleaf_item<TERMS>(uload(items + w), threadIdx.x & 63, insts, probs, leafs, spills, terms, parts);
+}
+
+// The forest pass of an energy call in one persistent launch: every
+// generation's chains, the deferred leaves and the energy reduction as
+// phases of one kernel, separated by grid barriers (grid_barrier), instead of
+// one launch each.  Each wave takes chain or leaf items from its XCD's share
+// of the phase (an atomic counter per share, then the other shares), so
+// instances stay on the XCD, and in the L2, that holds their sites, as in
+// the generation launches; the per-item code is theirs, so energies are
+// bitwise the same.  The grid holds only resident workgroups, sized by the
+// launcher from the kernel's occupancy.
+// The light phases of the persistent launch stay out of line, so that the
+// chain item inlined in its loop keeps the registers of the generation
+// launches' kernels.
+__device__ __attribute__((noinline)) void rleaf_item_call(const int2 it, const int lane,
+                                                          const InstDev *__restrict__ insts,
+                                                          const ProbDev *__restrict__ probs,
+                                                          const double2 *__restrict__ leafs,
+                                                          const double2 *__restrict__ spills,
+                                                          double *__restrict__ parts) {
+  rleaf_item<false>(it, lane, insts, probs, leafs, spills, nullptr, parts);
+}
+
+__device__ __attribute__((noinline)) void reduce_phase_call(const InstDev *__restrict__ insts, int ninst,
+                                                            const ProbDev *__restrict__ probs,
+                                                            const double *__restrict__ parts,
+                                                            double *__restrict__ energies, double *ssum,
+                                                            double *scomp) {
+  for (int i = blockIdx.x; i < ninst; i += gridDim.x) {
+    const InstDev inst = insts[i];
+    reduce_energy_block(inst, probs[inst.prob], parts, energies, 1, ssum, scomp);
+  }
+}
+
+template <bool IZ>
+__device__ __forceinline__ void rforest_persist_body(const PhaseDev *__restrict__ phases, int nphase,
+                                                     int *__restrict__ sync, const InstDev *__restrict__ insts,
+                                                     int ninst, const ProbDev *__restrict__ probs,
+                                                     const double *__restrict__ rarena,
+                                                     const double *__restrict__ rarenaT, double2 *__restrict__ leafs,
+                                                     double2 *__restrict__ spills, double *__restrict__ parts,
+                                                     double *__restrict__ energies, int *__restrict__ status) {
+  __shared__ double ssum[256], scomp[256];
+  const int lane = threadIdx.x & 63;
+  const int xcd = blockIdx.x & 7;
+#pragma nounroll
+  for (int ph = 0; ph < nphase; ++ph) {
+    // the phase's fields one word at a time (a copy of the whole struct, indexed
+    // by share below, would live in scratch)
+    const int kind = uload(&phases[ph].kind);
+    const int2 *items = reinterpret_cast<const int2 *>(uload(&phases[ph].items));
+    int *ctr = sync + 4 + 8 * ph;
+    if (kind == 2) {
+      reduce_phase_call(insts, ninst, probs, parts, energies, ssum, scomp);
+    } else {
+#pragma nounroll
+      for (int s = 0; s < 8; ++s) {
+        const int x = (xcd + s) & 7;
+        const int b0 = uload(&phases[ph].share[x]), n = uload(&phases[ph].share[x + 1]) - b0;
+        for (;;) {
+          int k = 0;
+          if (lane == 0) k = atomicAdd(ctr + x, 1);
+          k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+          if (k >= n) break;
+          const int2 it = uload(items + b0 + k);
+          if (kind == 0)
+            rchain_item<false, IZ>(it, lane, insts, probs, rarena, rarenaT, leafs, spills, nullptr, parts);
+          else
+            rleaf_item_call(it, lane, insts, probs, leafs, spills, parts);
+        }
+      }
+    }
+    if (ph + 1 < nphase) grid_barrier(sync, ctr, status, ninst);
   }
 }
 
@@ -675,6 +760,26 @@ k_rchains_energy_iz(const int2 *__restrict__ items, int nitems, const InstDev *_
   rchains_body<false, true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
 }
 
+extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
+k_rforest_persist(const PhaseDev *__restrict__ phases, int nphase, int *__restrict__ sync,
+                  const InstDev *__restrict__ insts, int ninst, const ProbDev *__restrict__ probs,
+                  const double *__restrict__ rarena, long long nsite, double2 *__restrict__ leafs,
+                  double2 *__restrict__ spills, double *__restrict__ parts, double *__restrict__ energies,
+                  int *__restrict__ status) {
+  rforest_persist_body<false>(phases, nphase, sync, insts, ninst, probs, rarena, rarena + nsite, leafs, spills, parts,
+                              energies, status);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_IZ_WAVES
+k_rforest_persist_iz(const PhaseDev *__restrict__ phases, int nphase, int *__restrict__ sync,
+                     const InstDev *__restrict__ insts, int ninst, const ProbDev *__restrict__ probs,
+                     const double *__restrict__ rarena, long long nsite, double2 *__restrict__ leafs,
+                     double2 *__restrict__ spills, double *__restrict__ parts, double *__restrict__ energies,
+                     int *__restrict__ status) {
+  rforest_persist_body<true>(phases, nphase, sync, insts, ninst, probs, rarena, rarena + nsite, leafs, spills, parts,
+                             energies, status);
+}
+
 extern "C" __global__ void __launch_bounds__(256)
 k_rleaf_terms(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
               const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
@@ -729,6 +834,27 @@ hipError_t kl_rchains_bwd(int nitems, int wpb, bool iz, hipStream_t s, const int
 hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                            const double2 *aslot, double2 *abar) {
   hipLaunchKernelGGL(k_rsite_reduce, dim3(max_n, ninst), dim3(256), 0, s, insts, probs, aslot, abar);
+  return hipGetLastError();
+}
+
+// Resident workgroups of the persistent forest kernel: its occupancy per CU
+// times the CUs, a multiple of 8 (workgroup b runs on XCD b % 8).
+int persist_grid(bool iz, int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(iz ? k_rforest_persist_iz
+                                                                                              : k_rforest_persist),
+                                                   256, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return 0;
+  return (per_cu * cus) & ~7;
+}
+
+hipError_t kl_rforest_persist(int grid, bool iz, hipStream_t s, const PhaseDev *phases, int nphase, int *sync,
+                              const InstDev *insts, int ninst, const ProbDev *probs, const double *rarena,
+                              long long nsite, double2 *leafs, double2 *spills, double *parts, double *energies,
+                              int *status) {
+  hipLaunchKernelGGL(iz ? k_rforest_persist_iz : k_rforest_persist, dim3(grid), dim3(256), 0, s, phases, nphase, sync,
+                     insts, ninst, probs, rarena, nsite, leafs, spills, parts, energies, status);
   return hipGetLastError();
 }
 

@@ -102,6 +102,29 @@ def test_graph_replay_matches_eager(uneven):
     assert torch.equal(e, e2) and torch.equal(b, b2)
 
 
+@pytest.mark.skip(reason="launch order 2 (the forest pass as one persistent launch: generations, deferred leaves and the energy reduction as phases behind grid barriers bounded by 0.2 s of wall clock) did not return within the test's 120 s on its first GPU call (40-qubit batch of 256, 4 waves per group); reading the kernel, its barrier and pop loop and their ISA showed no cause, so set_launch rejects order 2")
+def test_persistent_launch_evaluate_and_replay(uneven):
+    """The forest pass in one persistent launch (order 2) inside evaluate and
+    inside the captured graph: energies and bitstrings bitwise equal to the
+    generation launches, on every replay."""
+    import torch
+    parts, probs, x, sd = uneven
+    print('case: uneven-64 evaluate, persistent launch, graph capture + 3 replays', flush=True)
+    batch = Batch(probs, [1] * len(probs))
+    e0, b0, _ = batch.evaluate(x, SHOTS, sd)
+    e0, b0 = e0.clone(), b0.clone()
+    batch.set_launch(4, 2)
+    e, b, _ = batch.evaluate(x, SHOTS, sd)
+    assert torch.equal(e, e0) and torch.equal(b, b0)
+    g = batch.capture(x, SHOTS, sd)
+    for rep in range(3):
+        e, b, _ = g.replay()
+        assert torch.equal(e, e0), f'replay {rep}'
+        assert torch.equal(b, b0), f'replay {rep}'
+        batch.energies(x)  # eager persistent launches between replays
+        torch.cuda.synchronize()
+
+
 def test_autotuner_shapes_do_not_change_results(uneven):
     import torch
     parts, probs, x, sd = uneven
