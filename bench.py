@@ -126,6 +126,29 @@ def cpu_baselines(budget_s, with_grad):
     return out
 
 
+def timed_calls(fn, steps, batch):
+    """`steps` calls of fn with the batch's HIP-event instrumentation off (the
+    wall time: throughput), then `steps` more with it on (per-phase and
+    per-launch device times: the breakdown and the roofline).  The events
+    recorded around every launch cost GPU time of their own, so the two are
+    measured apart.  Returns (last result, seconds per call, phase_ms, calls)."""
+    import torch
+    batch.set_timing(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = fn()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    batch.set_timing(True)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    ph, calls = batch.timing()
+    batch.set_timing(False)
+    return r, dt, ph, calls
+
+
 def measured_traffic(key='hbm_bytes_per_launch:'):
     """HBM bytes per launch of the chain kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
@@ -178,15 +201,9 @@ def main():
     x = torch.tensor(w['x'], dtype=torch.float64, device='cuda')
     for _ in range(args.warmup):
         e = batch.energies(x)
-    torch.cuda.synchronize()
-    batch.set_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e = batch.energies(x)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    phase_ms, calls = batch.timing()
-    batched_rate = B * args.steps / dt
+    e, dt_call, phase_ms, calls = timed_calls(lambda: batch.energies(x), args.steps, batch)
+    dt = dt_call * args.steps
+    batched_rate = B / dt_call
 
     # plain comparator: the same kernels, one circuit per launch sequence
     plain_rate, same = plain_rate_of(lambda b, v: b.energies(v), Batch([prob], [1]), x[: args.plain_sample],
@@ -225,16 +242,9 @@ def main():
     complex_leg = None
     if real:
         batch.set_arith('complex')
-        e_cx = batch.energies(x)
-        torch.cuda.synchronize()
-        batch.set_timing(False)
-        batch.set_timing(True)
-        t1 = time.perf_counter()
-        for _ in range(args.steps):
-            e_cx = batch.energies(x)
-        torch.cuda.synchronize()
-        cx_rate = B * args.steps / (time.perf_counter() - t1)
-        cx_ms, cx_calls = batch.timing()
+        batch.energies(x)
+        e_cx, cx_dt, cx_ms, cx_calls = timed_calls(lambda: batch.energies(x), args.steps, batch)
+        cx_rate = B / cx_dt
         cx_chains = cx_ms[2] / max(cx_calls, 1)
         cx_kern = cx_ms[6] / max(cx_calls, 1)
         cx_flops = info['flops_energy_executed'] * 0.75 * B / (cx_kern * 1e-3) / 1e12
@@ -246,7 +256,6 @@ def main():
                        "kernel": ("k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy")
                                  + " + k_leaf_energy (complex128, Gauss-form MFMA products)"}
         batch.set_arith('auto')
-        batch.set_timing(False)
 
     # the same energies with the state built by exact CNOT bond doubling (no
     # SVD; the reverse engine's representation) instead of gates + SVDs
@@ -262,16 +271,9 @@ def main():
     # the same energies with the pivoted QR / SVD run at every gate, although
     # no split of this workload cuts rank (the default takes the exact split)
     batch.set_prep('factorize')
-    e_fac = batch.energies(x)
-    torch.cuda.synchronize()
-    batch.set_timing(True)
-    t3 = time.perf_counter()
-    for _ in range(args.steps):
-        e_fac = batch.energies(x)
-    torch.cuda.synchronize()
-    fac_rate = B * args.steps / (time.perf_counter() - t3)
-    fac_ms, fac_calls = batch.timing()
-    batch.set_timing(False)
+    batch.energies(x)
+    e_fac, fac_dt, fac_ms, fac_calls = timed_calls(lambda: batch.energies(x), args.steps, batch)
+    fac_rate = B / fac_dt
     fac_dev = float(torch.max(torch.abs(e_fac - e) / torch.clamp(torch.abs(e), min=1.0)))
     batch.set_prep('svd')
 
@@ -301,6 +303,8 @@ def main():
                             "state_prep_ms": round(fac_ms[0] / max(fac_calls, 1), 3),
                             "note": "same energies, pivoted QR (SVD gauge) at every gate instead of the exact split"},
         "exact_split": bool(info['exact_split']),
+        "timing": "value and ms_per_step: uninstrumented calls; phase_ms_per_step and roofline: as many calls "
+                  "again with HIP events around every launch on the launch stream",
         "phase_ms_per_step": {"state_prep": round(phase_ms[0] / max(calls, 1), 3),
                               "forest_chains": round(chains_ms, 3),
                               "chain_kernel": round(kern_ms_step, 3),
@@ -434,14 +438,7 @@ def bench_sampling(args):
         real = all(p.info()['real_arith'] for p in probs)
         shots = 1000
         b.sample(x, shots, sd)
-        torch.cuda.synchronize()
-        b.set_timing(True)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            b.sample(x, shots, sd)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
-        ph, calls = b.timing()
+        _, dt, ph, calls = timed_calls(lambda: b.sample(x, shots, sd), reps, b)
         c = max(calls, 1)
         envs_ms, sample_ms = ph[1] / c, ph[2] / c
         flops = sum(workcount.sampling_flops(workcount.allocated_bonds(q['n'], q['ops'])[0], shots) for q in parts)
@@ -501,15 +498,9 @@ def bench_grads(args, prob, info, x, B):
     batch.set_checkpoint(args.grad_checkpoint)
     for _ in range(args.warmup):
         batch.value_grad(x)
-    torch.cuda.synchronize()
-    batch.set_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(args.grad_steps):
-        e, g = batch.value_grad(x)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    phase_ms, calls = batch.timing()
-    rate = B * args.grad_steps / dt
+    (e, g), dt_call, phase_ms, calls = timed_calls(lambda: batch.value_grad(x), args.grad_steps, batch)
+    dt = dt_call * args.grad_steps
+    rate = B / dt_call
     del batch
     torch.cuda.empty_cache()
     single = Batch([prob], [1])
