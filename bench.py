@@ -175,8 +175,9 @@ def main():
     ap.add_argument('--no-uneven', action='store_true')
     ap.add_argument('--uneven-reps', type=int, default=5)
     ap.add_argument('--grad-steps', type=int, default=3)
-    ap.add_argument('--grad-checkpoint', type=int, default=1,
-                    help='40-qubit adjoint: keep every k-th forward step environment (the rest are recomputed)')
+    ap.add_argument('--grad-checkpoint', type=int, default=0,
+                    help='40-qubit adjoint: keep every k-th forward step environment (the rest are recomputed); '
+                         '0 (default): the gradient autotuner picks it with the workgroup size')
     ap.add_argument('--plain-grad-sample', type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -491,11 +492,19 @@ def bench_grads(args, prob, info, x, B):
     """Adjoint gradients (value + dE/dx) of the same 256 vectors: one batched
     reverse pass per step vs one vector per call."""
     import torch
-    from gvmps import Batch
+    from gvmps import Batch, autotune
     batch = Batch([prob], [B])
     # every k-th forward step keeps its input environment; the backward pass
-    # recomputes the rest (bitwise the same gradients at every stride)
-    batch.set_checkpoint(args.grad_checkpoint)
+    # recomputes the rest (bitwise the same gradients at every stride); by
+    # default the autotuner times strides 1, 2 x 4, 2 waves per workgroup
+    tuned = None
+    if args.grad_checkpoint > 0:
+        batch.set_checkpoint(args.grad_checkpoint)
+        stride, wpb = args.grad_checkpoint, 4
+    else:
+        (wpb, stride), table = autotune.tune_grad(batch, x, reps=2, use_cache=False)
+        tuned = {"choice": {"waves_per_group": wpb, "checkpoint_stride": stride},
+                 "ms": {f"{k[0]}w/stride{k[1]}": round(v, 3) for k, v in table.items()}}
     for _ in range(args.warmup):
         batch.value_grad(x)
     (e, g), dt_call, phase_ms, calls = timed_calls(lambda: batch.value_grad(x), args.grad_steps, batch)
@@ -504,7 +513,7 @@ def bench_grads(args, prob, info, x, B):
     del batch
     torch.cuda.empty_cache()
     single = Batch([prob], [1])
-    single.set_checkpoint(args.grad_checkpoint)
+    single.set_checkpoint(stride)
     plain_rate, same = plain_rate_of(
         lambda b, v: b.value_grad(v), single, x[: args.plain_grad_sample],
         lambda i, r: torch.equal(r[0], e[i:i + 1]) and torch.equal(r[1], g.view(B, -1)[i]))
@@ -525,7 +534,9 @@ def bench_grads(args, prob, info, x, B):
         "plain_matches_batched_bitwise": same,
         "phase_ms_per_step": {n: round(phase_ms[i] / c, 3) for i, n in enumerate(names)},
         "arithmetic": "real" if real else "complex",
-        "checkpoint_stride": args.grad_checkpoint,
+        "checkpoint_stride": stride,
+        "waves_per_group": wpb,
+        "autotuned": tuned,
         "chains_tflops": round(achieved, 3),
         "chains_frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 4),
         "backward_kernel_ms_per_step": round(phase_ms[6] / c, 3),

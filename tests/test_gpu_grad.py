@@ -295,3 +295,30 @@ def test_checkpoint_strides_are_bitwise_identical(cuda):
             e, g = b.value_grad(X)
             assert np.array_equal(e.cpu().numpy(), e1), (arith, stride)
             assert np.array_equal(g.cpu().numpy(), g1), (arith, stride)
+
+
+def test_gradient_autotuner_configs_are_bitwise_identical(cuda):
+    """Every launch shape and checkpoint stride the gradient autotuner tries
+    gives bitwise the same energies and gradients, and the tuner leaves the
+    batch on its choice."""
+    from gvmps import autotune
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], **SVD)
+    X = w['x'][:8]
+    b = Batch([p], [8])
+    print('case: spsa-40q x 8, gradient autotuner configurations', flush=True)
+    ref = None
+    for wpb, stride in autotune.GRAD_CONFIGS:
+        b.set_launch(wpb, 1)
+        b.set_checkpoint(stride)
+        e, g = b.value_grad(X)
+        e, g = e.cpu().numpy(), g.cpu().numpy()
+        if ref is None:
+            ref = (e, g)
+        else:
+            assert np.array_equal(e, ref[0]) and np.array_equal(g, ref[1]), (wpb, stride)
+    best, times = autotune.tune_grad(b, X, reps=1, use_cache=False)
+    assert best in autotune.GRAD_CONFIGS and len(times) == len(autotune.GRAD_CONFIGS)
+    assert b.launch == (best[0], 1)
+    e, g = b.value_grad(X)
+    assert np.array_equal(g.cpu().numpy(), ref[1])
