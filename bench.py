@@ -217,13 +217,15 @@ def main():
     chains_ms = phase_ms[2] / max(calls, 1)
     kern_ms, kern_launches = phase_ms[6], int(phase_ms[7])
     kern_ms_step = kern_ms / max(calls, 1)
-    # executed MFMA flops of the chain kernel: the planner counts (4 or 6)
-    # chi^3 complex multiply-adds per expansion step at 8 flops each; the real
-    # path does one real product per complex product (a quarter of that), the
-    # complex path three real products (Gauss form, three quarters)
+    # executed MFMA flops of the chain kernel, counted by the library per step
+    # of the last call: the products each step issues (one real product per
+    # complex product on the real path; fewer where the sites' zero structure
+    # lets a step skip all-zero row groups or merge its two products)
     real = bool(info['real_arith'])
-    exec_per_eval = info['flops_energy_executed'] * (0.25 if real else 0.75)
-    achieved = exec_per_eval * B / (kern_ms_step * 1e-3) / 1e12
+    exec_per_step = batch.chain_flops()
+    exec_per_eval = exec_per_step / B
+    full_per_eval = info['flops_energy_executed'] * (0.25 if real else 0.75)
+    achieved = exec_per_step / (kern_ms_step * 1e-3) / 1e12
     # the algorithm's own work: left/right environments with shared prefixes,
     # counted from the workload (gvmps.workcount); the kernels do less
     # arithmetic than that (real state, prefix and suffix forests with
@@ -233,6 +235,7 @@ def main():
         "flops_per_eval": wc['flops_per_eval'], "node_site_transfers": wc['node_site_transfers'],
         "reference_chunk128_dag_flops_per_eval": wc['reference_dag_flops_per_eval'],
         "executed_flops_per_eval": exec_per_eval,
+        "executed_flops_per_eval_without_zero_skips": full_per_eval,
         "reduction_vs_algorithmic": round(wc['flops_per_eval'] / exec_per_eval, 2),
         "basis": "16 (Dl^2 Dr + Dl Dr^2) flops per environment transfer, N_s distinct prefixes per site over "
                  "the words' spans, plus the left and right identity environments",
@@ -248,7 +251,7 @@ def main():
         cx_rate = B / cx_dt
         cx_chains = cx_ms[2] / max(cx_calls, 1)
         cx_kern = cx_ms[6] / max(cx_calls, 1)
-        cx_flops = info['flops_energy_executed'] * 0.75 * B / (cx_kern * 1e-3) / 1e12
+        cx_flops = batch.chain_flops() / (cx_kern * 1e-3) / 1e12
         complex_leg = {"evals_per_s": round(cx_rate, 2), "forest_chains_ms": round(cx_chains, 3),
                        "chain_kernel_ms": round(cx_kern, 3),
                        "achieved_tflops": round(cx_flops, 3), "frac": round(cx_flops / PEAK_FP64_TFLOPS, 4),
@@ -319,10 +322,12 @@ def main():
                                else ("k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy"),
                      "launches_per_step": kern_launches // max(calls, 1),
                      "avg_launch_us": round(kern_ms / max(kern_launches, 1) * 1e3, 1),
-                     "flops_basis": "executed MFMA flops of the kernel per step: " + (
-                         "one real 16x16x16 product per complex product (real state)" if real else
+                     "flops_basis": "executed MFMA flops of the kernel per step, counted by the library "
+                                    "(gvmps_batch_chain_flops): " + (
+                         "one real 16x16x16 product per complex product (real state), fewer where a step "
+                         "skips all-zero row groups of a site or merges its two products" if real else
                          "three real products per complex product (Gauss form)"),
-                     "executed_flops_per_step": exec_per_eval * B},
+                     "executed_flops_per_step": exec_per_step},
         "algorithmic": algorithmic,
         "complex_path": complex_leg,
     }

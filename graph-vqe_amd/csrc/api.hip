@@ -275,7 +275,10 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   B->d_inst = upload(B->owned, B->inst.data(), B->inst.size());
   B->arena = upload<double2>(B->owned, nullptr, 2 * site);
   B->real = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->real_ok; });
-  if (B->real) B->rarena = upload<double>(B->owned, nullptr, 2 * site);
+  if (B->real) {
+    B->rarena = upload<double>(B->owned, nullptr, 2 * site);
+    B->smask = upload<unsigned>(B->owned, nullptr, 2 * (size_t)(site / GV_SLOT));
+  }
   B->iz = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->iz_only; });
   B->envs = upload<double2>(B->owned, nullptr, env);
   B->spills = upload<double2>(B->owned, nullptr, spill);
@@ -454,6 +457,8 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
   if (B.real && B.arith == 0) {
     // real circuits, no Y words: the same forests in real arithmetic
     hip_check(kl_pack_real(B.n_site_total, stream, B.arena, B.rarena), "k_pack_real launch");
+    if (B.use_smask)
+      hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask), "k_site_masks launch");
     if (B.order == 2 && !terms) {
       // one persistent launch: generations, leaves and reduction as phases
       if (B.phases_dirty) build_phases(B);
@@ -473,7 +478,8 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
       B.kbegin(stream);
-      hip_check(kl_rchains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total, B.leafs,
+      hip_check(kl_rchains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+                           B.use_smask ? B.smask : nullptr, B.leafs,
                            B.spills,
                            terms ? B.terms : nullptr, B.parts),
                 "k_rchains launch");
@@ -990,6 +996,53 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
     B.wpb = waves_per_group;
     B.order = order;
     B.phases_dirty = true;
+  })
+}
+
+int gvmps_batch_chain_flops(gvmps_batch *b, double *flops) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (!flops) throw std::runtime_error("null output");
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    std::vector<unsigned> mask;
+    const bool real = B.real && B.arith == 0;
+    if (real && B.use_smask) {
+      mask.resize(2 * (size_t)(B.n_site_total / GV_SLOT));
+      hip_check(hipMemcpy(mask.data(), B.smask, mask.size() * sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    // FP64 flops of one real 16x16x16 product; the complex kernels execute three
+    // (Gauss form) per complex product
+    const double prod = 2.0 * GV_CHI * GV_CHI * GV_CHI;
+    double total = 0.0;
+    for (const InstDev &in : B.inst) {
+      const Problem &P = *B.probs[in.prob];
+      const long long slot0 = in.site_off / GV_SLOT;
+      for (const ChainDev &ch : P.chains)
+        for (int st = 0; st < ch.nsteps; ++st) {
+          const StepDev &sd = P.steps[ch.step0 + st];
+          const bool both = sd.need == 3;
+          double products;  // real 16^3 products of the step
+          if (!real) {
+            products = 3.0 * (both ? 6.0 : 4.0);
+          } else if (mask.empty()) {
+            products = both ? 6.0 : 4.0;
+          } else {
+            const unsigned mw = mask[2 * (slot0 + sd.site) + (ch.dir != 0 ? 1 : 0)];
+            if (mw & 0x100u) products = 2.0;                        // merged pair serves both passes
+            else if ((mw & 0xffu) == 0xc3u) products = both ? 3.0 : 2.0;  // row split: half k-steps
+            else products = both ? 6.0 : 4.0;
+          }
+          total += products * prod;
+        }
+    }
+    *flops = total;
+  })
+}
+
+int gvmps_batch_set_sparsity(gvmps_batch *b, int on) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    B.use_smask = on != 0;
   })
 }
 

@@ -59,6 +59,8 @@ struct Batch {
   bool iz = false;    // every problem's forest has I/Z children only: k_rchains*_iz
   int arith = 0;              // 0: real path when `real`, 1: complex path always
   double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
+  unsigned *smask = nullptr;  // zero structure of each real site slot, direct and transposed (k_site_masks)
+  bool use_smask = true;      // chain steps skip exact-zero products (gvmps_batch_set_sparsity)
   // persistent forest launch (order 2): phase table, barrier and pop words
   PhaseDev *d_phases = nullptr;
   int nphase = 0;
@@ -135,8 +137,9 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 // kernel launchers (kernels_real.hip)
 hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena);
 hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
-                      const ProbDev *probs, const double *rarena, long long nsite, double2 *leafs, double2 *spills,
-                      double2 *terms, double *parts);
+                      const ProbDev *probs, const double *rarena, long long nsite, const unsigned *smask,
+                      double2 *leafs, double2 *spills, double2 *terms, double *parts);
+hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 int persist_grid(bool iz, int device);

@@ -124,6 +124,53 @@ __device__ __forceinline__ d4 rprod_y(const d4 &x, const RFrag &f, int j) {
   return y;
 }
 
+// Zero structure of a site, per (instance, site, copy), from k_site_masks:
+//   bits 0-3 / 4-7   row groups t (rows 4t..4t+3) of M_0 / M_1 with a nonzero
+//   bit 8            the column supports of M_0 and M_1 are disjoint
+//   bits 16-31       the columns of M_1
+// Both products of a step contract over the rows of M_i in k-steps of four
+// rows, so a k-step over an all-zero row group only adds exact zeros and can
+// be skipped; the chain steps do so for the row split of bulk sites (M_0 in
+// rows 0-7, M_1 in rows 8-15: groups 0-1 and 2-3), in straight-line code (a
+// branch per k-step would break the MFMA issue).  With disjoint columns, X = E (M_0 + M_1) holds X_0 = E M_0 in
+// the columns of M_0 and X_1 in those of M_1, and Y = (M_0 + M_1)^T X holds
+// M_a^T X_b in the block (columns of M_a) x (columns of M_b): one product
+// pair gives Y00, Y11, Y01 and Y10, each entry from the same products in the
+// same order as the separate products (the rest being exact zeros).
+#define GV_SM_DISJOINT 0x100u
+// Products over the k-steps in the compile-time set G (row groups of M_i).
+template <unsigned G>
+__device__ __forceinline__ d4 rprod_x_g(const REnv &e, const RFrag &f, int i) {
+  d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (G & (1u << t)) x = mfma(e.v[t], f.v[i][t], x);
+  return x;
+}
+
+template <unsigned G>
+__device__ __forceinline__ d4 rprod_y_g(const d4 &x, const RFrag &f, int j) {
+  d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (G & (1u << t)) y = mfma(x[t], f.v[j][t], y);
+  return y;
+}
+
+// Y = F^T (E F) for F = M_0 + M_1 (disjoint columns), E layout.
+__device__ __forceinline__ d4 rprod_merged(const REnv &e, const RFrag &f) {
+  double F[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) F[t] = f.v[0][t] + f.v[1][t];
+  d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) x = mfma(e.v[t], F[t], x);
+  d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) y = mfma(x[t], F[t], y);
+  return y;
+}
+
 // One generation of prefix/suffix chains in real arithmetic; the same work
 // items and slot numbering as chains_body in kernels_energy.hip.  IZ: every
 // step of the forest has I/Z children only (Hamiltonians without X words), so
@@ -133,13 +180,16 @@ __device__ __forceinline__ d4 rprod_y(const d4 &x, const RFrag &f, int j) {
 template <bool TERMS, bool IZ>
 __device__ __forceinline__ void rchain_item(const int2 it, const int lane, const InstDev *__restrict__ insts,
                                             const ProbDev *__restrict__ probs, const double *__restrict__ rarena,
-                                            const double *__restrict__ rarenaT, double2 *__restrict__ leafs,
-                                            double2 *__restrict__ spills, double2 *__restrict__ terms,
-                                            double *__restrict__ parts) {
+                                            const double *__restrict__ rarenaT, const unsigned *__restrict__ smask,
+                                            double2 *__restrict__ leafs, double2 *__restrict__ spills,
+                                            double2 *__restrict__ terms, double *__restrict__ parts) {
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
   const double *sites = (ch.dir != 0 ? rarenaT : rarena) + inst.site_off;
+  // this instance's zero-structure words for the copy the chain reads
+  const unsigned *sm = smask ? smask + (inst.site_off / GV_SLOT) * 2 + (ch.dir != 0 ? 1 : 0) : nullptr;
+  const int a16 = lane & 15, g4 = lane >> 4;
   double2 *spill = spills + inst.spill_off;
   double2 *tv = terms + inst.term_off;
 
@@ -158,14 +208,75 @@ __device__ __forceinline__ void rchain_item(const int2 it, const int lane, const
   auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
     nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));
     rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
-    const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
+    const unsigned mw = sm ? uload(sm + 2 * sd.site) : 0xffu;
+    const bool merged = (mw & GV_SM_DISJOINT) != 0;
+    const bool rowsplit = !merged && (mw & 0xffu) == 0xc3u;  // M_0 rows 0-7, M_1 rows 8-15
+    // Y_ab of the merged product ym is its block (columns of M_a) x (columns
+    // of M_b): pass 0 takes (0,0) and (1,1), pass 1 (the X children) (0,1) and (1,0)
+    auto split = [&](const d4 &ym, int pass, d4 &ya, d4 &yb) {
+      const unsigned c1 = mw >> 16;
+      const unsigned ra = (c1 >> a16) & 1u;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const unsigned cb = ((c1 >> kdx(g4, v)) & 1u) ^ (unsigned)pass;
+        ya[v] = (ra == 0u && cb == 0u) ? ym[v] : 0.0;
+        yb[v] = (ra == 1u && cb == 1u) ? ym[v] : 0.0;
+      }
+    };
+    d4 x0, x1, ym;
+    if (IZ) {
+      // the I/Z form keeps only (ya, yb) live past this point.  One branch
+      // per step into straight-line MFMA sequences: the merged pair (over
+      // all row groups; zero groups add exact zeros), the two-group row
+      // split of bulk sites (M_0 in rows 0-7, M_1 in rows 8-15), or the
+      // full products.
+      if (merged) {
+        split(rprod_merged(e, f), 0, x0, x1);
+      } else if (rowsplit) {
+        d4 p0 = {0.0, 0.0, 0.0, 0.0}, p1 = {0.0, 0.0, 0.0, 0.0};
+        p0 = mfma(e.v[0], f.v[0][0], p0);
+        p1 = mfma(e.v[2], f.v[1][2], p1);
+        p0 = mfma(e.v[1], f.v[0][1], p0);
+        p1 = mfma(e.v[3], f.v[1][3], p1);
+        d4 q0 = {0.0, 0.0, 0.0, 0.0}, q1 = {0.0, 0.0, 0.0, 0.0};
+        q0 = mfma(p0[0], f.v[0][0], q0);
+        q1 = mfma(p1[2], f.v[1][2], q1);
+        q0 = mfma(p0[1], f.v[0][1], q0);
+        q1 = mfma(p1[3], f.v[1][3], q1);
+        x0 = q0;
+        x1 = q1;
+      } else {
+        const d4 p0 = rprod_x(e, f, 0), p1 = rprod_x(e, f, 1);
+        x0 = rprod_y(p0, f, 0);
+        x1 = rprod_y(p1, f, 1);
+      }
+    } else if (merged) {
+      ym = rprod_merged(e, f);
+    } else if (rowsplit) {
+      x0 = rprod_x_g<3u>(e, f, 0);
+      x1 = rprod_x_g<12u>(e, f, 1);
+    } else {
+      x0 = rprod_x(e, f, 0);
+      x1 = rprod_x(e, f, 1);
+    }
     REnv cont;
     // pass 0: I/Z children from Y00, Y11; pass 1: X children from Y01, Y10
 #pragma nounroll
     for (int pass = 0; pass < (IZ ? 1 : 2); ++pass) {
       if (!IZ && !(sd.need & (1 << pass))) continue;
-      const d4 ya = rprod_y(pass ? x1 : x0, f, 0);
-      const d4 yb = rprod_y(pass ? x0 : x1, f, 1);
+      d4 ya, yb;
+      if (IZ) {  // formed above: x0 = Y00, x1 = Y11
+        ya = x0;
+        yb = x1;
+      } else if (merged) {
+        split(ym, pass, ya, yb);
+      } else if (rowsplit) {
+        ya = rprod_y_g<3u>(pass ? x1 : x0, f, 0);
+        yb = rprod_y_g<12u>(pass ? x0 : x1, f, 1);
+      } else {
+        ya = rprod_y(pass ? x1 : x0, f, 0);
+        yb = rprod_y(pass ? x0 : x1, f, 1);
+      }
       for (int c = 0; c < sd.nchild; ++c) {
         const ChildDev cd = uload(P.children + sd.child0 + c);
         if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
@@ -213,12 +324,13 @@ template <bool TERMS, bool IZ>
 __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int nitems,
                                              const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
                                              const double *__restrict__ rarena, const double *__restrict__ rarenaT,
-                                             double2 *__restrict__ leafs, double2 *__restrict__ spills,
-                                             double2 *__restrict__ terms, double *__restrict__ parts) {
+                                             const unsigned *__restrict__ smask, double2 *__restrict__ leafs,
+                                             double2 *__restrict__ spills, double2 *__restrict__ terms,
+                                             double *__restrict__ parts) {
   const int w = xcd_item();
   if (w >= nitems) return;
-  rchain_item<TERMS, IZ>(uload(items + w), threadIdx.x & 63, insts, probs, rarena, rarenaT, leafs, spills, terms,
-                         parts);
+  rchain_item<TERMS, IZ>(uload(items + w), threadIdx.x & 63, insts, probs, rarena, rarenaT, smask, leafs, spills,
+                         terms, parts);
 }
 
 template <bool TERMS>
@@ -329,7 +441,7 @@ __device__ __forceinline__ void rforest_persist_body(const PhaseDev *__restrict_
           if (k >= n) break;
           const int2 it = uload(items + b0 + k);
           if (kind == 0)
-            rchain_item<false, IZ>(it, lane, insts, probs, rarena, rarenaT, leafs, spills, nullptr, parts);
+            rchain_item<false, IZ>(it, lane, insts, probs, rarena, rarenaT, nullptr, leafs, spills, nullptr, parts);
           else
             rleaf_item_call(it, lane, insts, probs, leafs, spills, parts);
         }
@@ -734,30 +846,34 @@ k_pack_real(const double2 *__restrict__ arena, double *__restrict__ rarena, doub
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
           const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                    const unsigned *__restrict__ smask,
           double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  rchains_body<true, false>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, terms, nullptr);
+  rchains_body<true, false>(items, nitems, insts, probs, rarena, rarena + nsite, smask, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                  const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                    const unsigned *__restrict__ smask,
                  double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
-  rchains_body<false, false>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
+  rchains_body<false, false>(items, nitems, insts, probs, rarena, rarena + nsite, smask, leafs, spills, nullptr, parts);
 }
 
 // Forests whose steps have I/Z children only.
 extern "C" __global__ void __launch_bounds__(256) GV_FIVE_WAVES
 k_rchains_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
              const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                    const unsigned *__restrict__ smask,
              double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
-  rchains_body<true, true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, terms, nullptr);
+  rchains_body<true, true>(items, nitems, insts, probs, rarena, rarena + nsite, smask, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_IZ_WAVES
 k_rchains_energy_iz(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                     const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                    const unsigned *__restrict__ smask,
                     double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
-  rchains_body<false, true>(items, nitems, insts, probs, rarena, rarena + nsite, leafs, spills, nullptr, parts);
+  rchains_body<false, true>(items, nitems, insts, probs, rarena, rarena + nsite, smask, leafs, spills, nullptr, parts);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
@@ -803,15 +919,61 @@ hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double
 }
 
 hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
-                      const ProbDev *probs, const double *rarena, long long nsite, double2 *leafs, double2 *spills,
-                      double2 *terms, double *parts) {
+                      const ProbDev *probs, const double *rarena, long long nsite, const unsigned *smask,
+                      double2 *leafs, double2 *spills, double2 *terms, double *parts) {
   const dim3 grid((nitems + wpb - 1) / wpb), block(64 * wpb);
   if (terms)
     hipLaunchKernelGGL(iz ? k_rchains_iz : k_rchains, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite,
-                       leafs, spills, terms);
+                       smask, leafs, spills, terms);
   else
     hipLaunchKernelGGL(iz ? k_rchains_energy_iz : k_rchains_energy, grid, block, 0, s, items, nitems, insts, probs,
-                       rarena, nsite, leafs, spills, parts);
+                       rarena, nsite, smask, leafs, spills, parts);
+  return hipGetLastError();
+}
+
+// Zero structure of every site (one wave per site slot) from the real arena
+// A (element (l, i, r) at (l*2 + i)*16 + r): for the direct copy (M_i = A_i)
+// and the transposed one (M_i = A_i^T), the row groups of M_i with a nonzero,
+// whether M_0 and M_1 have disjoint column supports, and the columns of M_1
+// (bit layout at GV_SM_DISJOINT).
+extern "C" __global__ void __launch_bounds__(256)
+k_site_masks(long long nslots, const double *__restrict__ rarena, unsigned *__restrict__ smask) {
+  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nslots) return;
+  const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+  const double *A = rarena + w * GV_SLOT;
+  unsigned rows[2] = {0u, 0u}, cols[2] = {0u, 0u};  // supports of A_i: rows l, columns r
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const unsigned long long b = __ballot(A[((g + 4 * t) * 2 + i) * 16 + j] != 0.0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((b >> (16 * q)) & 0xffffull) rows[i] |= 1u << (q + 4 * t);
+      cols[i] |= (unsigned)((b | (b >> 16) | (b >> 32) | (b >> 48)) & 0xffffull);
+    }
+  auto groups = [](unsigned supp) {
+    unsigned m = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if ((supp >> (4 * t)) & 15u) m |= 1u << t;
+    return m;
+  };
+  // direct copy: rows of M_i are rows of A_i, columns are columns; transposed: swapped
+  const unsigned direct = groups(rows[0]) | (groups(rows[1]) << 4) |
+                          ((cols[0] & cols[1]) == 0u ? GV_SM_DISJOINT : 0u) | (cols[1] << 16);
+  const unsigned transposed = groups(cols[0]) | (groups(cols[1]) << 4) |
+                              ((rows[0] & rows[1]) == 0u ? GV_SM_DISJOINT : 0u) | (rows[1] << 16);
+  if (lane == 0) {
+    smask[2 * w] = direct;
+    smask[2 * w + 1] = transposed;
+  }
+}
+
+hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask) {
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_site_masks, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, s, nslots, rarena, smask);
   return hipGetLastError();
 }
 

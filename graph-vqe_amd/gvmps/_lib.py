@@ -63,6 +63,8 @@ _PROTOS = {
     'gvmps_batch_set_prep': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_set_arith': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_set_checkpoint': [ctypes.c_void_p, ctypes.c_int],
+    'gvmps_batch_set_sparsity': [ctypes.c_void_p, ctypes.c_int],
+    'gvmps_batch_chain_flops': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
     'gvmps_batch_evaluate': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_capture': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

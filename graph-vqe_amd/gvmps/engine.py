@@ -163,6 +163,18 @@ class Batch:
         check(lib().gvmps_batch_set_launch(self._h, int(waves_per_group), int(order)))
         self.launch = (int(waves_per_group), int(order))
 
+    def set_sparsity(self, on=True):
+        """Real energy forests skip exact-zero products of the sites' zero
+        structure (default on); energies are the same either way."""
+        check(lib().gvmps_batch_set_sparsity(self._h, int(bool(on))))
+
+    def chain_flops(self):
+        """FP64 flops the forest-chain launches of the last energy call executed
+        (the products each step issued, zero-structure skips counted)."""
+        out = ctypes.c_double()
+        check(lib().gvmps_batch_chain_flops(self._h, ctypes.byref(out)))
+        return out.value
+
     def set_checkpoint(self, stride):
         """value_grad keeps every stride-th step environment of a chain and
         recomputes the rest in the backward pass (same results)."""

@@ -157,6 +157,27 @@ int gvmps_batch_value_grad(gvmps_batch *b, const double *params_dev, double *ene
  * engine keeps autograd's whole tape instead (test/tools/reverse_mps.py:86-99). */
 int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride);
 
+/* Zero-structure use of the real energy forests (default on): after the
+ * state is prepared, one pass records for every site which 4-row groups of
+ * A_i (and of A_i^T) hold a nonzero and whether the two physical components
+ * have disjoint column supports; each chain step then skips the k-steps of
+ * its products over all-zero row groups, and where the columns are disjoint
+ * it forms X = E (M_0 + M_1) and Y = (M_0 + M_1)^T X once instead of the
+ * four products X_i = E M_i, Y_ab = M_a^T X_b (Y_ab is a block of Y).  Only
+ * exact zeros are dropped, so energies are the same with it off (tested).
+ * The states of CNOT-entangled ansatze have this structure: in the 40-qubit
+ * workload every bulk site's A_0 and A_1 occupy disjoint halves of the left
+ * bond.  No reference counterpart (simulator_mps_dist.cu contracts dense
+ * sites). */
+int gvmps_batch_set_sparsity(gvmps_batch *b, int on);
+
+/* FP64 flops the forest-chain launches of the batch's last energy call
+ * executed, summed over its instances: per chain step the products the
+ * kernel issues (full, row-split or merged, from the sites' zero structure
+ * of that call; three real products per complex product on the complex
+ * path).  For the roofline of the chain kernel; no reference counterpart. */
+int gvmps_batch_chain_flops(gvmps_batch *b, double *flops);
+
 /* Launch configuration of the forest-chain kernels: wavefronts per workgroup
  * (1, 2 or 4) and work-item order (0 instance-major, 1 (default) longest
  * chains first within each XCD's share of the instance-major items, so each

@@ -295,6 +295,59 @@ def test_persistent_forest_launch_bitwise(cuda):
     assert rel_close(e1[0], d.energy(words, coeffs))
 
 
+def test_zero_structure_skipping_is_exact(cuda):
+    """The chain steps that skip all-zero row groups and merge the two
+    products where the sites' components have disjoint columns give the
+    energies and per-term values of the full products, bitwise: on the
+    40-qubit workload (I/Z forest; its bulk sites are column-disjoint in the
+    transposed copy, row-split in the direct one), on real circuits with X
+    words (general forest: the merged product's off-diagonal blocks feed the
+    X children) for linear, reverse-linear and random CNOT patterns, and on
+    an uneven multi-problem batch."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], max_bond=16, abs_cut=0, rel_cut=0)
+    b = Batch([p], [64])
+    print('case: 40q x 64, zero-structure skipping on/off', flush=True)
+    on = b.energies(w['x'][:64]).cpu().numpy()
+    fl_on = b.chain_flops()
+    b.set_sparsity(False)
+    off = b.energies(w['x'][:64]).cpu().numpy()
+    assert np.array_equal(on, off)
+    # executed-flop count: the planner's full count without skips; with them,
+    # every bulk step issues half its products (only sites 0-4 keep full ones)
+    full = p.info()['flops_energy_executed'] * 0.25 * 64
+    assert b.chain_flops() == pytest.approx(full, rel=1e-12)
+    assert 0.5 * full < fl_on < 0.6 * full
+    r = np.random.default_rng(21)
+    probs, X, tapes = [], [], []
+    for n, pattern in ((12, 'linear'), (15, 'reverse_linear'), (10, 'linear')):
+        ops, P = workloads.su2_ansatz(n, 4, ('ry',), pattern)
+        if n == 10:  # random neighbouring CNOT directions between the RY layers
+            ops = [o for o in ops if o[0] != 'cx']
+            for _ in range(30):
+                a = int(r.integers(0, n - 1))
+                ops.append(('cx', (a, a + 1) if r.random() < 0.5 else (a + 1, a), None))
+        words = [''.join(r.choice(list('IIXZ'), n)) for _ in range(200)] + ['I' * n]
+        q = Problem(n, ops, P, words, r.normal(size=len(words)), max_bond=16, abs_cut=0, rel_cut=0)
+        assert q.info()['real_arith'] == 1 and q.info()['iz_forest'] == 0
+        probs.append(q)
+        tapes.append(ops)
+        X.append(r.uniform(-np.pi, np.pi, (4, P)))
+    x = np.concatenate([xx.reshape(-1) for xx in X])
+    print('case: 3 real problems with X words, batch of 4 each, skipping on/off', flush=True)
+    bu = Batch(probs, [4, 4, 4])
+    e_on, t_on = bu.energies(x, terms=True)
+    e_on, t_on = e_on.cpu().numpy(), t_on.cpu().numpy()
+    bu.set_sparsity(False)
+    e_off, t_off = bu.energies(x, terms=True)
+    assert np.array_equal(e_on, e_off.cpu().numpy())
+    assert np.array_equal(t_on, t_off.cpu().numpy())
+    for j, (q, ops, xx) in enumerate(zip(probs, tapes, X)):
+        for k in range(4):
+            d = dense.Dense(q.n).run(tape_with_values(ops, xx[k]))
+            assert rel_close(e_on[4 * j + k], d.energy(q.words, q.coeffs))
+
+
 def test_bond_doubling_preparation_matches_svd(cuda):
     """Preparation mode 'doubling' (exact CNOT bond doubling, no SVD) gives the
     same energies (1e-10 relative) and the same samples as the SVD path."""
