@@ -278,6 +278,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
   if (B->real) {
     B->rarena = upload<double>(B->owned, nullptr, 2 * site);
     B->smask = upload<unsigned>(B->owned, nullptr, 2 * (size_t)(site / GV_SLOT));
+    B->rfarena = upload<double>(B->owned, nullptr, 2 * (size_t)(site / GV_SLOT) * GV_CHI * GV_CHI);
   }
   B->iz = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->iz_only; });
   B->envs = upload<double2>(B->owned, nullptr, env);
@@ -458,7 +459,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     // real circuits, no Y words: the same forests in real arithmetic
     hip_check(kl_pack_real(B.n_site_total, stream, B.arena, B.rarena), "k_pack_real launch");
     if (B.use_smask)
-      hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask), "k_site_masks launch");
+      hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask, B.rfarena), "k_site_masks launch");
     if (B.order == 2 && !terms) {
       // one persistent launch: generations, leaves and reduction as phases
       if (B.phases_dirty) build_phases(B);
@@ -479,7 +480,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
       if (ni == 0) continue;
       B.kbegin(stream);
       hip_check(kl_rchains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
-                           B.use_smask ? B.smask : nullptr, B.leafs,
+                           B.use_smask ? B.smask : nullptr, B.rfarena, B.leafs,
                            B.spills,
                            terms ? B.terms : nullptr, B.parts),
                 "k_rchains launch");

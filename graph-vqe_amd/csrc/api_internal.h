@@ -60,6 +60,7 @@ struct Batch {
   int arith = 0;              // 0: real path when `real`, 1: complex path always
   double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
   unsigned *smask = nullptr;  // zero structure of each real site slot, direct and transposed (k_site_masks)
+  double *rfarena = nullptr;  // merged fragments A_0 + A_1 of each slot, direct then transposed (k_site_masks)
   bool use_smask = true;      // chain steps skip exact-zero products (gvmps_batch_set_sparsity)
   // persistent forest launch (order 2): phase table, barrier and pop words
   PhaseDev *d_phases = nullptr;
@@ -138,8 +139,8 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena);
 hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                       const ProbDev *probs, const double *rarena, long long nsite, const unsigned *smask,
-                      double2 *leafs, double2 *spills, double2 *terms, double *parts);
-hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask);
+                      const double *farena, double2 *leafs, double2 *spills, double2 *terms, double *parts);
+hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask, double *farena);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 int persist_grid(bool iz, int device);

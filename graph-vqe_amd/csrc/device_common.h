@@ -32,6 +32,10 @@ __device__ __forceinline__ constexpr int kdx(int g, int t) { return g + 4 * t; }
 // the Y products, and contract one inline term end at a time: they fit six
 // waves per SIMD (80 registers; measured 2% faster than five, 4% than four).
 #define GV_IZ_WAVES __attribute__((amdgpu_waves_per_eu(6, 6)))
+// The same over merged site fragments (4 doubles per lane instead of 8).
+#ifndef GV_IZM_WAVES
+#define GV_IZM_WAVES __attribute__((amdgpu_waves_per_eu(6, 6)))
+#endif
 // The per-term form keeps more live state: five waves.
 // Complex chains of I/Z-only forests (k_chains*_iz).
 #ifndef GV_CIZ_WAVES

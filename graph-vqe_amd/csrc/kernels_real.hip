@@ -171,6 +171,59 @@ __device__ __forceinline__ d4 rprod_merged(const REnv &e, const RFrag &f) {
   return y;
 }
 
+// Y_ab of a merged product ym is its block (columns of M_a) x (columns of
+// M_b), the columns of M_1 being c1 = mw >> 16: pass 0 takes (0,0) and (1,1)
+// (the I/Z children), pass 1 (0,1) and (1,0) (the X children).
+__device__ __forceinline__ void rsplit(const d4 &ym, unsigned mw, int pass, int a16, int g4, d4 &ya, d4 &yb) {
+  const unsigned c1 = mw >> 16;
+  const unsigned ra = (c1 >> a16) & 1u;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const unsigned cb = ((c1 >> kdx(g4, v)) & 1u) ^ (unsigned)pass;
+    ya[v] = (ra == 0u && cb == 0u) ? ym[v] : 0.0;
+    yb[v] = (ra == 1u && cb == 1u) ? ym[v] : 0.0;
+  }
+}
+
+// The children of one step pass from (ya, yb) = (Y00, Y11) (pass 0) or
+// (Y01, Y10) (pass 1): I/X children take ya + yb, Z children ya - yb; each is
+// stored, deferred, contracted with its inline term ends, or continues the
+// chain (cont).
+template <bool TERMS, bool IZ>
+__device__ __forceinline__ void rstep_children(const ProbDev &P, const InstDev &inst, const StepDev &sd, int pass,
+                                               const d4 &ya, const d4 &yb, int lane, double2 *__restrict__ spill,
+                                               double2 *__restrict__ leafs, double2 *__restrict__ tv, double &acc,
+                                               REnv &cont) {
+  for (int c = 0; c < sd.nchild; ++c) {
+    const ChildDev cd = uload(P.children + sd.child0 + c);
+    if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
+    REnv ce;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
+    if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
+    if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
+    const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
+#pragma unroll(IZ ? GV_IZ_TERM_UNROLL : 2)
+    for (int t = 0; t < nterm_inline; ++t) {
+      const TermEnd te = uload(P.term_list + cd.term0 + t);
+      REnv r;
+      if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
+      else renv_unit(lane, r);  // term ends at site n-1
+      double sr = 0.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sr += ce.v[v] * r.v[v];
+      if (TERMS) {
+        sr = wave_sum(sr);
+        if (lane == 0) tv[te.term] = make_double2(sr, 0.0);
+      } else {
+        acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c <P>), <P> real
+      }
+    }
+    if (cd.action == -2) cont = ce;
+    else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
+  }
+}
+
 // One generation of prefix/suffix chains in real arithmetic; the same work
 // items and slot numbering as chains_body in kernels_energy.hip.  IZ: every
 // step of the forest has I/Z children only (Hamiltonians without X words), so
@@ -188,7 +241,7 @@ __device__ __forceinline__ void rchain_item(const int2 it, const int lane, const
   const ChainDev ch = uload(P.chains + it.y);
   const double *sites = (ch.dir != 0 ? rarenaT : rarena) + inst.site_off;
   // this instance's zero-structure words for the copy the chain reads
-  const unsigned *sm = smask ? smask + (inst.site_off / GV_SLOT) * 2 + (ch.dir != 0 ? 1 : 0) : nullptr;
+  const unsigned *sm = smask && !IZ ? smask + (inst.site_off / GV_SLOT) * 2 + (ch.dir != 0 ? 1 : 0) : nullptr;
   const int a16 = lane & 15, g4 = lane >> 4;
   double2 *spill = spills + inst.spill_off;
   double2 *tv = terms + inst.term_off;
@@ -211,45 +264,14 @@ __device__ __forceinline__ void rchain_item(const int2 it, const int lane, const
     const unsigned mw = sm ? uload(sm + 2 * sd.site) : 0xffu;
     const bool merged = (mw & GV_SM_DISJOINT) != 0;
     const bool rowsplit = !merged && (mw & 0xffu) == 0xc3u;  // M_0 rows 0-7, M_1 rows 8-15
-    // Y_ab of the merged product ym is its block (columns of M_a) x (columns
-    // of M_b): pass 0 takes (0,0) and (1,1), pass 1 (the X children) (0,1) and (1,0)
-    auto split = [&](const d4 &ym, int pass, d4 &ya, d4 &yb) {
-      const unsigned c1 = mw >> 16;
-      const unsigned ra = (c1 >> a16) & 1u;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const unsigned cb = ((c1 >> kdx(g4, v)) & 1u) ^ (unsigned)pass;
-        ya[v] = (ra == 0u && cb == 0u) ? ym[v] : 0.0;
-        yb[v] = (ra == 1u && cb == 1u) ? ym[v] : 0.0;
-      }
-    };
+    auto split = [&](const d4 &ym, int pass, d4 &ya, d4 &yb) { rsplit(ym, mw, pass, a16, g4, ya, yb); };
     d4 x0, x1, ym;
     if (IZ) {
-      // the I/Z form keeps only (ya, yb) live past this point.  One branch
-      // per step into straight-line MFMA sequences: the merged pair (over
-      // all row groups; zero groups add exact zeros), the two-group row
-      // split of bulk sites (M_0 in rows 0-7, M_1 in rows 8-15), or the
-      // full products.
-      if (merged) {
-        split(rprod_merged(e, f), 0, x0, x1);
-      } else if (rowsplit) {
-        d4 p0 = {0.0, 0.0, 0.0, 0.0}, p1 = {0.0, 0.0, 0.0, 0.0};
-        p0 = mfma(e.v[0], f.v[0][0], p0);
-        p1 = mfma(e.v[2], f.v[1][2], p1);
-        p0 = mfma(e.v[1], f.v[0][1], p0);
-        p1 = mfma(e.v[3], f.v[1][3], p1);
-        d4 q0 = {0.0, 0.0, 0.0, 0.0}, q1 = {0.0, 0.0, 0.0, 0.0};
-        q0 = mfma(p0[0], f.v[0][0], q0);
-        q1 = mfma(p1[2], f.v[1][2], q1);
-        q0 = mfma(p0[1], f.v[0][1], q0);
-        q1 = mfma(p1[3], f.v[1][3], q1);
-        x0 = q0;
-        x1 = q1;
-      } else {
-        const d4 p0 = rprod_x(e, f, 0), p1 = rprod_x(e, f, 1);
-        x0 = rprod_y(p0, f, 0);
-        x1 = rprod_y(p1, f, 1);
-      }
+      // the I/Z form keeps only (ya, yb) live past this point (the masked
+      // I/Z steps run in rchain_item_merged)
+      const d4 p0 = rprod_x(e, f, 0), p1 = rprod_x(e, f, 1);
+      x0 = rprod_y(p0, f, 0);
+      x1 = rprod_y(p1, f, 1);
     } else if (merged) {
       ym = rprod_merged(e, f);
     } else if (rowsplit) {
@@ -277,34 +299,7 @@ __device__ __forceinline__ void rchain_item(const int2 it, const int lane, const
         ya = rprod_y(pass ? x1 : x0, f, 0);
         yb = rprod_y(pass ? x0 : x1, f, 1);
       }
-      for (int c = 0; c < sd.nchild; ++c) {
-        const ChildDev cd = uload(P.children + sd.child0 + c);
-        if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
-        REnv ce;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
-        if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
-        if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
-        const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
-#pragma unroll(IZ ? GV_IZ_TERM_UNROLL : 2)
-        for (int t = 0; t < nterm_inline; ++t) {
-          const TermEnd te = uload(P.term_list + cd.term0 + t);
-          REnv r;
-          if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
-          else renv_unit(lane, r);  // term ends at site n-1
-          double sr = 0.0;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) sr += ce.v[v] * r.v[v];
-          if (TERMS) {
-            sr = wave_sum(sr);
-            if (lane == 0) tv[te.term] = make_double2(sr, 0.0);
-          } else {
-            acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c <P>), <P> real
-          }
-        }
-        if (cd.action == -2) cont = ce;
-        else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
-      }
+      rstep_children<TERMS, IZ>(P, inst, sd, pass, ya, yb, lane, spill, leafs, tv, acc, cont);
     }
     e = cont;
   };
@@ -331,6 +326,127 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
   if (w >= nitems) return;
   rchain_item<TERMS, IZ>(uload(items + w), threadIdx.x & 63, insts, probs, rarena, rarenaT, smask, leafs, spills,
                          terms, parts);
+}
+
+// Merged site fragments.  Where the two components of a site have disjoint
+// supports (both zero-structure cases above), F = M_0 + M_1 holds them both:
+// the merged products read F, and so does the row split (rows 0-7 of F are
+// those of M_0, rows 8-15 those of M_1).  k_site_masks writes F for every
+// slot and copy, row-major 16x16 (GV_FSLOT doubles), so an I/Z chain step
+// loads 2 KiB instead of 4 and keeps 4 doubles per lane per fragment instead
+// of 8; the few steps whose site has no such structure (bonds below 16 at
+// the chain ends) load the full slot when they run.
+#define GV_FSLOT (GV_CHI * GV_CHI)
+
+__device__ __forceinline__ void ffrag_load(const double *slot, int lane, double (&f)[4]) {
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) f[t] = slot[kdx(g, t) * 16 + j];
+}
+
+template <bool TERMS>
+__device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane, const InstDev *__restrict__ insts,
+                                                   const ProbDev *__restrict__ probs,
+                                                   const double *__restrict__ rarena, const double *__restrict__ rarenaT,
+                                                   const double *__restrict__ farena,
+                                                   const double *__restrict__ farenaT,
+                                                   const unsigned *__restrict__ smask, double2 *__restrict__ leafs,
+                                                   double2 *__restrict__ spills, double2 *__restrict__ terms,
+                                                   double *__restrict__ parts) {
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
+  const long long slot0 = inst.site_off / GV_SLOT;
+  const double *sites = (ch.dir != 0 ? rarenaT : rarena) + inst.site_off;
+  const double *fsites = (ch.dir != 0 ? farenaT : farena) + slot0 * GV_FSLOT;
+  const unsigned *sm = smask + slot0 * 2 + (ch.dir != 0 ? 1 : 0);
+  const int a16 = lane & 15, g4 = lane >> 4;
+  double2 *spill = spills + inst.spill_off;
+  double2 *tv = terms + inst.term_off;
+
+  REnv e;
+  if (ch.src >= 0) renv_load_sym(rslot(spill, ch.src), lane, e);
+  else renv_unit(lane, e);
+  double acc = 0.0;
+
+  StepDev sd0 = uload(P.steps + ch.step0);
+  double fa[4], fb[4];
+  ffrag_load(fsites + (size_t)sd0.site * GV_FSLOT, lane, fa);
+  // unconditional prefetch of the next step's merged fragment, as in rchain_item
+  auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const double(&f)[4], double(&nf)[4]) {
+    nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));
+    ffrag_load(fsites + (size_t)nsd.site * GV_FSLOT, lane, nf);
+    const unsigned mw = uload(sm + 2 * sd.site);
+    d4 y0, y1;
+    if (mw & GV_SM_DISJOINT) {
+      d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) x = mfma(e.v[t], f[t], x);
+      d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) y = mfma(x[t], f[t], y);
+      rsplit(y, mw, 0, a16, g4, y0, y1);
+    } else if ((mw & 0xffu) == 0xc3u) {  // M_0 rows 0-7 (f[0], f[1]), M_1 rows 8-15 (f[2], f[3])
+      d4 p0 = {0.0, 0.0, 0.0, 0.0}, p1 = {0.0, 0.0, 0.0, 0.0};
+      p0 = mfma(e.v[0], f[0], p0);
+      p1 = mfma(e.v[2], f[2], p1);
+      p0 = mfma(e.v[1], f[1], p0);
+      p1 = mfma(e.v[3], f[3], p1);
+      d4 q0 = {0.0, 0.0, 0.0, 0.0}, q1 = {0.0, 0.0, 0.0, 0.0};
+      q0 = mfma(p0[0], f[0], q0);
+      q1 = mfma(p1[2], f[2], q1);
+      q0 = mfma(p0[1], f[1], q0);
+      q1 = mfma(p1[3], f[3], q1);
+      y0 = q0;
+      y1 = q1;
+    } else {  // no structure: the components from the full slot, one at a time
+      const double *slot = sites + (size_t)sd.site * GV_SLOT;
+      double m[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) m[t] = slot[(kdx(g4, t) * 2 + i) * 16 + a16];
+        d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x = mfma(e.v[t], m[t], x);
+        d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) y = mfma(x[t], m[t], y);
+        if (i == 0) y0 = y;
+        else y1 = y;
+      }
+    }
+    REnv cont;
+    rstep_children<TERMS, true>(P, inst, sd, 0, y0, y1, lane, spill, leafs, tv, acc, cont);
+    e = cont;
+  };
+  for (int st = 0; st < ch.nsteps; st += 2) {
+    StepDev sd1;
+    run_step(st, sd0, sd1, fa, fb);
+    if (st + 1 >= ch.nsteps) break;
+    run_step(st + 1, sd1, sd0, fb, fa);
+  }
+  if (!TERMS) {
+    acc = wave_sum(acc);
+    if (lane == 0) parts[inst.part_off + it.y] = acc;
+  }
+}
+
+template <bool TERMS>
+__device__ __forceinline__ void rchains_merged_body(const int2 *__restrict__ items, int nitems,
+                                                    const InstDev *__restrict__ insts,
+                                                    const ProbDev *__restrict__ probs,
+                                                    const double *__restrict__ rarena,
+                                                    const double *__restrict__ rarenaT,
+                                                    const double *__restrict__ farena,
+                                                    const double *__restrict__ farenaT,
+                                                    const unsigned *__restrict__ smask, double2 *__restrict__ leafs,
+                                                    double2 *__restrict__ spills, double2 *__restrict__ terms,
+                                                    double *__restrict__ parts) {
+  const int w = xcd_item();
+  if (w >= nitems) return;
+  rchain_item_merged<TERMS>(uload(items + w), threadIdx.x & 63, insts, probs, rarena, rarenaT, farena, farenaT, smask,
+                            leafs, spills, terms, parts);
 }
 
 template <bool TERMS>
@@ -876,6 +992,25 @@ k_rchains_energy_iz(const int2 *__restrict__ items, int nitems, const InstDev *_
   rchains_body<false, true>(items, nitems, insts, probs, rarena, rarena + nsite, smask, leafs, spills, nullptr, parts);
 }
 
+// I/Z forests over the merged fragments (zero-structure masks on).
+extern "C" __global__ void __launch_bounds__(256) GV_FIVE_WAVES
+k_rchains_izm(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+              const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+              const double *__restrict__ farena, long long nslots, const unsigned *__restrict__ smask,
+              double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  rchains_merged_body<true>(items, nitems, insts, probs, rarena, rarena + nsite, farena, farena + nslots * GV_FSLOT,
+                            smask, leafs, spills, terms, nullptr);
+}
+
+extern "C" __global__ void __launch_bounds__(256) GV_IZM_WAVES
+k_rchains_energy_izm(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                     const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
+                     const double *__restrict__ farena, long long nslots, const unsigned *__restrict__ smask,
+                     double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
+  rchains_merged_body<false>(items, nitems, insts, probs, rarena, rarena + nsite, farena, farena + nslots * GV_FSLOT,
+                             smask, leafs, spills, nullptr, parts);
+}
+
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rforest_persist(const PhaseDev *__restrict__ phases, int nphase, int *__restrict__ sync,
                   const InstDev *__restrict__ insts, int ninst, const ProbDev *__restrict__ probs,
@@ -920,8 +1055,18 @@ hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double
 
 hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                       const ProbDev *probs, const double *rarena, long long nsite, const unsigned *smask,
-                      double2 *leafs, double2 *spills, double2 *terms, double *parts) {
+                      const double *farena, double2 *leafs, double2 *spills, double2 *terms, double *parts) {
   const dim3 grid((nitems + wpb - 1) / wpb), block(64 * wpb);
+  const long long nslots = nsite / GV_SLOT;
+  if (iz && smask) {  // I/Z forests over the merged fragments
+    if (terms)
+      hipLaunchKernelGGL(k_rchains_izm, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite, farena, nslots,
+                         smask, leafs, spills, terms);
+    else
+      hipLaunchKernelGGL(k_rchains_energy_izm, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite, farena,
+                         nslots, smask, leafs, spills, parts);
+    return hipGetLastError();
+  }
   if (terms)
     hipLaunchKernelGGL(iz ? k_rchains_iz : k_rchains, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite,
                        smask, leafs, spills, terms);
@@ -937,22 +1082,34 @@ hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *i
 // whether M_0 and M_1 have disjoint column supports, and the columns of M_1
 // (bit layout at GV_SM_DISJOINT).
 extern "C" __global__ void __launch_bounds__(256)
-k_site_masks(long long nslots, const double *__restrict__ rarena, unsigned *__restrict__ smask) {
+k_site_masks(long long nslots, const double *__restrict__ rarena, unsigned *__restrict__ smask,
+             double *__restrict__ farena) {
   const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= nslots) return;
   const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
   const double *A = rarena + w * GV_SLOT;
   unsigned rows[2] = {0u, 0u}, cols[2] = {0u, 0u};  // supports of A_i: rows l, columns r
+  double f[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const unsigned long long b = __ballot(A[((g + 4 * t) * 2 + i) * 16 + j] != 0.0);
+      const double a = A[((g + 4 * t) * 2 + i) * 16 + j];
+      f[t] += a;
+      const unsigned long long b = __ballot(a != 0.0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if ((b >> (16 * q)) & 0xffffull) rows[i] |= 1u << (q + 4 * t);
       cols[i] |= (unsigned)((b | (b >> 16) | (b >> 32) | (b >> 48)) & 0xffffull);
     }
+  // merged fragments F = A_0 + A_1 (direct copy) and its transpose
+  // (transposed copy), row-major; meaningful where the supports are disjoint
+  double *fd = farena + w * GV_FSLOT, *ft = farena + (nslots + w) * GV_FSLOT;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    fd[(g + 4 * t) * 16 + j] = f[t];
+    ft[j * 16 + g + 4 * t] = f[t];
+  }
   auto groups = [](unsigned supp) {
     unsigned m = 0;
 #pragma unroll
@@ -971,9 +1128,10 @@ k_site_masks(long long nslots, const double *__restrict__ rarena, unsigned *__re
   }
 }
 
-hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask) {
+hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask, double *farena) {
   if (nslots <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_site_masks, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, s, nslots, rarena, smask);
+  hipLaunchKernelGGL(k_site_masks, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, s, nslots, rarena, smask,
+                     farena);
   return hipGetLastError();
 }
 
