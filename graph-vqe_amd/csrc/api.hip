@@ -254,6 +254,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       gleaf += P.ngleaf;
       B->max_n = std::max(B->max_n, P.n);
       B->max_params = std::max(B->max_params, P.nparams);
+      B->max_chains = std::max(B->max_chains, (int)P.chains.size());
       site += (long long)P.n * GV_SLOT;
       env += 2LL * (P.n + 1) * GV_ENV;
       spill += (long long)P.nspill * GV_ENV;
@@ -279,6 +280,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     B->rarena = upload<double>(B->owned, nullptr, 2 * site);
     B->smask = upload<unsigned>(B->owned, nullptr, 2 * (size_t)(site / GV_SLOT));
     B->rfarena = upload<double>(B->owned, nullptr, 2 * (size_t)(site / GV_SLOT) * GV_CHI * GV_CHI);
+    B->cflags = upload<int>(B->owned, nullptr, std::max(1LL, part));
   }
   B->iz = std::all_of(B->probs.begin(), B->probs.end(), [](const Problem *P) { return P->iz_only; });
   B->envs = upload<double2>(B->owned, nullptr, env);
@@ -458,8 +460,11 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
   if (B.real && B.arith == 0) {
     // real circuits, no Y words: the same forests in real arithmetic
     hip_check(kl_pack_real(B.n_site_total, stream, B.arena, B.rarena), "k_pack_real launch");
-    if (B.use_smask)
+    if (B.use_smask) {
       hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask, B.rfarena), "k_site_masks launch");
+      hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags),
+                "k_chain_flags launch");
+    }
     if (B.order == 2 && !terms) {
       // one persistent launch: generations, leaves and reduction as phases
       if (B.phases_dirty) build_phases(B);
@@ -480,7 +485,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
       if (ni == 0) continue;
       B.kbegin(stream);
       hip_check(kl_rchains(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
-                           B.use_smask ? B.smask : nullptr, B.rfarena, B.leafs,
+                           B.use_smask ? B.smask : nullptr, B.rfarena, B.cflags, B.leafs,
                            B.spills,
                            terms ? B.terms : nullptr, B.parts),
                 "k_rchains launch");
@@ -1006,10 +1011,20 @@ int gvmps_batch_chain_flops(gvmps_batch *b, double *flops) {
     if (!flops) throw std::runtime_error("null output");
     hip_check(hipSetDevice(B.device), "hipSetDevice");
     std::vector<unsigned> mask;
+    std::vector<int> flag;  // I/Z forests: chains on the merged fragments (the rest run full products)
     const bool real = B.real && B.arith == 0;
     if (real && B.use_smask) {
       mask.resize(2 * (size_t)(B.n_site_total / GV_SLOT));
       hip_check(hipMemcpy(mask.data(), B.smask, mask.size() * sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
+      if (B.iz) {
+        long long nparts = 0;
+        for (const InstDev &in : B.inst) {
+          const Problem &P = *B.probs[in.prob];
+          nparts = std::max(nparts, in.part_off + (long long)P.chains.size());
+        }
+        flag.resize(nparts);
+        hip_check(hipMemcpy(flag.data(), B.cflags, flag.size() * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+      }
     }
     // FP64 flops of one real 16x16x16 product; the complex kernels execute three
     // (Gauss form) per complex product
@@ -1018,14 +1033,16 @@ int gvmps_batch_chain_flops(gvmps_batch *b, double *flops) {
     for (const InstDev &in : B.inst) {
       const Problem &P = *B.probs[in.prob];
       const long long slot0 = in.site_off / GV_SLOT;
-      for (const ChainDev &ch : P.chains)
+      for (size_t c = 0; c < P.chains.size(); ++c) {
+        const ChainDev &ch = P.chains[c];
+        const bool full_chain = !flag.empty() && flag[in.part_off + c] == 0;
         for (int st = 0; st < ch.nsteps; ++st) {
           const StepDev &sd = P.steps[ch.step0 + st];
           const bool both = sd.need == 3;
           double products;  // real 16^3 products of the step
           if (!real) {
             products = 3.0 * (both ? 6.0 : 4.0);
-          } else if (mask.empty()) {
+          } else if (mask.empty() || full_chain) {
             products = both ? 6.0 : 4.0;
           } else {
             const unsigned mw = mask[2 * (slot0 + sd.site) + (ch.dir != 0 ? 1 : 0)];
@@ -1035,6 +1052,7 @@ int gvmps_batch_chain_flops(gvmps_batch *b, double *flops) {
           }
           total += products * prod;
         }
+      }
     }
     *flops = total;
   })

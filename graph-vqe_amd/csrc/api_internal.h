@@ -61,6 +61,7 @@ struct Batch {
   double *rarena = nullptr;   // real site arena: n_site_total doubles as A_i, then n_site_total as A_i^T
   unsigned *smask = nullptr;  // zero structure of each real site slot, direct and transposed (k_site_masks)
   double *rfarena = nullptr;  // merged fragments A_0 + A_1 of each slot, direct then transposed (k_site_masks)
+  int *cflags = nullptr;      // per (instance, chain): its steps run on the merged fragments (k_chain_flags)
   bool use_smask = true;      // chain steps skip exact-zero products (gvmps_batch_set_sparsity)
   // persistent forest launch (order 2): phase table, barrier and pop words
   PhaseDev *d_phases = nullptr;
@@ -75,7 +76,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   long long n_terms_total = 0, n_params_total = 0;
-  int max_n = 0, max_params = 0;
+  int max_n = 0, max_params = 0, max_chains = 0;
   // reverse gradient (allocated on first use)
   long long n_rot_total = 0, n_steps_total = 0, n_gleaf_total = 0, n_site_total = 0, n_spill_total = 0;
   bool grad_ready = false;
@@ -139,8 +140,11 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream);
 hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double *rarena);
 hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                       const ProbDev *probs, const double *rarena, long long nsite, const unsigned *smask,
-                      const double *farena, double2 *leafs, double2 *spills, double2 *terms, double *parts);
+                      const double *farena, const int *cflags, double2 *leafs, double2 *spills, double2 *terms,
+                      double *parts);
 hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask, double *farena);
+hipError_t kl_chain_flags(int ninst, int max_chains, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                          const unsigned *smask, int *cflags);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
                           const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
 int persist_grid(bool iz, int device);

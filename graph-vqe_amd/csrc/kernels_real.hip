@@ -432,6 +432,17 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
   }
 }
 
+// A site step has structure when its components are column-disjoint or row
+// split (the two cases the merged fragments serve).
+__device__ __forceinline__ bool site_structured(unsigned mw) {
+  return (mw & GV_SM_DISJOINT) != 0u || (mw & 0xffu) == 0xc3u;
+}
+
+// Chains with at most a quarter of their steps on sites without structure run
+// over the merged fragments (those steps load their full slots when they run);
+// the rest, e.g. every chain of a state prepared with a decomposition at each
+// gate, run the full-slot loop with its prefetch.  The flag sits at the
+// chain's partial-energy index.
 template <bool TERMS>
 __device__ __forceinline__ void rchains_merged_body(const int2 *__restrict__ items, int nitems,
                                                     const InstDev *__restrict__ insts,
@@ -440,13 +451,20 @@ __device__ __forceinline__ void rchains_merged_body(const int2 *__restrict__ ite
                                                     const double *__restrict__ rarenaT,
                                                     const double *__restrict__ farena,
                                                     const double *__restrict__ farenaT,
-                                                    const unsigned *__restrict__ smask, double2 *__restrict__ leafs,
+                                                    const unsigned *__restrict__ smask,
+                                                    const int *__restrict__ cflags, double2 *__restrict__ leafs,
                                                     double2 *__restrict__ spills, double2 *__restrict__ terms,
                                                     double *__restrict__ parts) {
   const int w = xcd_item();
   if (w >= nitems) return;
-  rchain_item_merged<TERMS>(uload(items + w), threadIdx.x & 63, insts, probs, rarena, rarenaT, farena, farenaT, smask,
-                            leafs, spills, terms, parts);
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  if (uload(cflags + inst.part_off + it.y))
+    rchain_item_merged<TERMS>(it, threadIdx.x & 63, insts, probs, rarena, rarenaT, farena, farenaT, smask, leafs,
+                              spills, terms, parts);
+  else
+    rchain_item<TERMS, true>(it, threadIdx.x & 63, insts, probs, rarena, rarenaT, nullptr, leafs, spills, terms,
+                             parts);
 }
 
 template <bool TERMS>
@@ -997,18 +1015,20 @@ extern "C" __global__ void __launch_bounds__(256) GV_FIVE_WAVES
 k_rchains_izm(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
               const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
               const double *__restrict__ farena, long long nslots, const unsigned *__restrict__ smask,
-              double2 *__restrict__ leafs, double2 *__restrict__ spills, double2 *__restrict__ terms) {
+              const int *__restrict__ cflags, double2 *__restrict__ leafs, double2 *__restrict__ spills,
+              double2 *__restrict__ terms) {
   rchains_merged_body<true>(items, nitems, insts, probs, rarena, rarena + nsite, farena, farena + nslots * GV_FSLOT,
-                            smask, leafs, spills, terms, nullptr);
+                            smask, cflags, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_IZM_WAVES
 k_rchains_energy_izm(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                      const ProbDev *__restrict__ probs, const double *__restrict__ rarena, long long nsite,
                      const double *__restrict__ farena, long long nslots, const unsigned *__restrict__ smask,
-                     double2 *__restrict__ leafs, double2 *__restrict__ spills, double *__restrict__ parts) {
+                     const int *__restrict__ cflags, double2 *__restrict__ leafs, double2 *__restrict__ spills,
+                     double *__restrict__ parts) {
   rchains_merged_body<false>(items, nitems, insts, probs, rarena, rarena + nsite, farena, farena + nslots * GV_FSLOT,
-                             smask, leafs, spills, nullptr, parts);
+                             smask, cflags, leafs, spills, nullptr, parts);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
@@ -1055,16 +1075,17 @@ hipError_t kl_pack_real(long long n, hipStream_t s, const double2 *arena, double
 
 hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                       const ProbDev *probs, const double *rarena, long long nsite, const unsigned *smask,
-                      const double *farena, double2 *leafs, double2 *spills, double2 *terms, double *parts) {
+                      const double *farena, const int *cflags, double2 *leafs, double2 *spills, double2 *terms,
+                      double *parts) {
   const dim3 grid((nitems + wpb - 1) / wpb), block(64 * wpb);
   const long long nslots = nsite / GV_SLOT;
   if (iz && smask) {  // I/Z forests over the merged fragments
     if (terms)
       hipLaunchKernelGGL(k_rchains_izm, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite, farena, nslots,
-                         smask, leafs, spills, terms);
+                         smask, cflags, leafs, spills, terms);
     else
       hipLaunchKernelGGL(k_rchains_energy_izm, grid, block, 0, s, items, nitems, insts, probs, rarena, nsite, farena,
-                         nslots, smask, leafs, spills, parts);
+                         nslots, smask, cflags, leafs, spills, parts);
     return hipGetLastError();
   }
   if (terms)
@@ -1132,6 +1153,30 @@ hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, 
   if (nslots <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_site_masks, dim3((unsigned)((nslots + 3) / 4)), dim3(256), 0, s, nslots, rarena, smask,
                      farena);
+  return hipGetLastError();
+}
+
+// Per (instance, chain): 1 when at most a quarter of the chain's steps are on
+// sites without structure (rchains_merged_body).
+extern "C" __global__ void __launch_bounds__(256)
+k_chain_flags(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+              const unsigned *__restrict__ smask, int *__restrict__ cflags) {
+  const InstDev inst = uload(insts + blockIdx.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (c >= P.nchains) return;
+  const ChainDev ch = P.chains[c];
+  const unsigned *sm = smask + (inst.site_off / GV_SLOT) * 2 + (ch.dir != 0 ? 1 : 0);
+  int plain = 0;
+  for (int s = 0; s < ch.nsteps; ++s) plain += site_structured(sm[2 * P.steps[ch.step0 + s].site]) ? 0 : 1;
+  cflags[inst.part_off + c] = 4 * plain <= ch.nsteps ? 1 : 0;
+}
+
+hipError_t kl_chain_flags(int ninst, int max_chains, hipStream_t s, const InstDev *insts, const ProbDev *probs,
+                          const unsigned *smask, int *cflags) {
+  if (ninst <= 0 || max_chains <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_chain_flags, dim3(ninst, (max_chains + 255) / 256), dim3(256), 0, s, insts, probs, smask,
+                     cflags);
   return hipGetLastError();
 }
 
