@@ -318,14 +318,15 @@ def main():
                      "traffic": traffic,
                      "traffic_basis": "HBM bytes per launch of the chain kernel (mean of its generation launches), "
                                       "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated",
-                     "kernel": ("k_rchains_energy_iz" if info['iz_forest'] else "k_rchains_energy") if real
+                     "kernel": ("k_rchains_energy_izm" if info['iz_forest'] else "k_rchains_energy") if real
                                else ("k_chains_energy_iz" if info['iz_forest'] else "k_chains_energy"),
                      "launches_per_step": kern_launches // max(calls, 1),
                      "avg_launch_us": round(kern_ms / max(kern_launches, 1) * 1e3, 1),
                      "flops_basis": "executed MFMA flops of the kernel per step, counted by the library "
                                     "(gvmps_batch_chain_flops): " + (
                          "one real 16x16x16 product per complex product (real state), fewer where a step "
-                         "skips all-zero row groups of a site or merges its two products" if real else
+                         "merges its two products or skips all-zero row groups of a site (merged site "
+                         "fragments)" if real else
                          "three real products per complex product (Gauss form)"),
                      "executed_flops_per_step": exec_per_step},
         "algorithmic": algorithmic,
