@@ -166,15 +166,15 @@ def measured_traffic(key='hbm_bytes_per_launch:'):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
-    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--plain-sample', type=int, default=32, help='circuits timed one per launch')
     ap.add_argument('--cpu-budget', type=float, default=15.0, help='seconds of CPU oracle work')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-grad', action='store_true')
     ap.add_argument('--no-uneven', action='store_true')
-    ap.add_argument('--uneven-reps', type=int, default=5)
-    ap.add_argument('--grad-steps', type=int, default=3)
+    ap.add_argument('--uneven-reps', type=int, default=20)
+    ap.add_argument('--grad-steps', type=int, default=10)
     ap.add_argument('--grad-checkpoint', type=int, default=0,
                     help='40-qubit adjoint: keep every k-th forward step environment (the rest are recomputed); '
                          '0 (default): the gradient autotuner picks it with the workgroup size')

@@ -32,7 +32,14 @@ using namespace gv;
                        // so that the 16 row pairs of a round start in different LDS banks
 #define MAX_SWEEPS 40
 
+// Pending single-site matrices of the first GV_PEND_LDS two-site gates,
+// computed by all threads of the workgroup before the gate schedule starts
+// (their rotations' sines and cosines are then off the schedule's dependent
+// path); later gates of longer tapes fold theirs when they run.
+#define GV_PEND_LDS 192
+
 struct PrepLds {
+  double2 pend[GV_PEND_LDS][8];  // Ul (4), Ur (4) of gate gi
   double2 W[GV_WAVES][32][PW];
   double nrm[GV_WAVES][32];
   int perm[GV_WAVES][32];
@@ -378,15 +385,23 @@ __device__ int wave_jacobi(double2 (*W)[PW], int rows, int nc, int width, int la
 // single-site matrices and Theta are real; the Theta products take one MFMA
 // product instead of four and the QR runs in real arithmetic.
 template <bool REAL>
-__device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const GateDev gd, const double *x,
-                         double2 *sites, int *status, bool exact) {
+__device__ void run_gate(PrepLds &L, int w, int lane, const ProbDev &P, const int gi, const GateDev gd,
+                         const double *x, double2 *sites, int *status, bool exact) {
   const unsigned long long t0 = clk();
   const int s = gd.site;
   const int Dl = L.bond[s], Dc = L.bond[s + 1], Dr = L.bond[s + 2];
   const int m = 2 * Dl, nc = 2 * Dr;
   double2 Ul[4], Ur[4];
-  pending_matrix(P.preops + gd.pre0, gd.npre_l, x, P.fixes, Ul);
-  pending_matrix(P.preops + gd.pre0 + gd.npre_l, gd.npre_r, x, P.fixes, Ur);
+  if (gi < GV_PEND_LDS) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      Ul[q] = L.pend[gi][q];
+      Ur[q] = L.pend[gi][4 + q];
+    }
+  } else {
+    pending_matrix(P.preops + gd.pre0, gd.npre_l, x, P.fixes, Ul);
+    pending_matrix(P.preops + gd.pre0 + gd.npre_l, gd.npre_r, x, P.fixes, Ur);
+  }
   const double2 *sa = sites + (size_t)s * GV_SLOT;
   const double2 *sb = sites + (size_t)(s + 1) * GV_SLOT;
   const int j16 = lane & 15, g = lane >> 4;
@@ -616,12 +631,22 @@ __device__ __forceinline__ void prep_body(const InstDev *__restrict__ insts, con
     L.sweeps = 0;
     L.cyc[0] = L.cyc[1] = L.cyc[2] = L.cyc[3] = 0;
   }
+  for (int gi = tid; gi < min(P.ngates, GV_PEND_LDS); gi += 64 * GV_WAVES) {
+    const GateDev gd = P.g2[gi];
+    double2 U[4];
+    pending_matrix(P.preops + gd.pre0, gd.npre_l, x, P.fixes, U);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) L.pend[gi][q] = U[q];
+    pending_matrix(P.preops + gd.pre0 + gd.npre_l, gd.npre_r, x, P.fixes, U);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) L.pend[gi][4 + q] = U[q];
+  }
   __syncthreads();
 
   const bool exact = P.exact_split && !factor_always;
   for (int row = 0; row < P.nsched; ++row) {
     const int gi = uload(P.sched + row * GV_WAVES + w);
-    if (gi >= 0) run_gate<REAL>(L, w, lane, P, uload(P.g2 + gi), x, sites, st, exact);
+    if (gi >= 0) run_gate<REAL>(L, w, lane, P, gi, uload(P.g2 + gi), x, sites, st, exact);
     __syncthreads();
   }
   // single-site ops left pending after the last two-site gate on each site
