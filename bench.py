@@ -433,7 +433,6 @@ def bench_sampling(args):
     import numpy as np
     import torch
     from gvmps import Batch, Problem, seeds, workcount, workloads
-    from oracle import mps_ref
     out = {}
     reps = 20
     for name, parts in (('plain-12q', [workloads.config(0)]), ('uneven-64', workloads.config(3)['parts'])):
@@ -461,6 +460,7 @@ def bench_sampling(args):
                                   "frac_of_fp64_peak": round(achieved / PEAK_FP64_TFLOPS, 5),
                                   "algorithmic_flops_per_call": flops}}
     if not args.no_cpu:
+        from oracle import mps_ref  # the CPU baseline only
         w = workloads.config(0)
         tape = [(g, q, (w['x'][0][p] if p is not None else None)) for g, q, p in w['ops']]
         sim = mps_ref.RefSimulator(w['n'], max_bond=16, abs_cut=0.0, rel_cut=0.0).run(tape)

@@ -1167,8 +1167,19 @@ k_chain_flags(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
   if (c >= P.nchains) return;
   const ChainDev ch = P.chains[c];
   const unsigned *sm = smask + (inst.site_off / GV_SLOT) * 2 + (ch.dir != 0 ? 1 : 0);
+  // eight steps per round: their descriptor loads, then their mask loads,
+  // are issued together instead of one dependent pair per step
   int plain = 0;
-  for (int s = 0; s < ch.nsteps; ++s) plain += site_structured(sm[2 * P.steps[ch.step0 + s].site]) ? 0 : 1;
+  for (int s0 = 0; s0 < ch.nsteps; s0 += 8) {
+    int site[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) site[q] = s0 + q < ch.nsteps ? P.steps[ch.step0 + s0 + q].site : -1;
+    unsigned mw[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) mw[q] = site[q] >= 0 ? sm[2 * site[q]] : GV_SM_DISJOINT;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) plain += site_structured(mw[q]) ? 0 : 1;
+  }
   cflags[inst.part_off + c] = 4 * plain <= ch.nsteps ? 1 : 0;
 }
 
