@@ -385,7 +385,19 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
       d4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int t = 0; t < 4; ++t) y = mfma(x[t], f[t], y);
-      rsplit(y, mw, 0, a16, g4, y0, y1);
+      if ((mw >> 16) == 0xff00u) {
+        // the bond-16 bulk: M_0 in columns 0-7, M_1 in 8-15, so Y00 is
+        // lanes a < 8 in registers 0-1 and Y11 lanes a >= 8 in registers
+        // 2-3 (the same selects as rsplit, with the bit tests folded)
+        const bool lo = a16 < 8;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          y0[v] = (v < 2 && lo) ? y[v] : 0.0;
+          y1[v] = (v >= 2 && !lo) ? y[v] : 0.0;
+        }
+      } else {
+        rsplit(y, mw, 0, a16, g4, y0, y1);
+      }
     } else if ((mw & 0xffu) == 0xc3u) {  // M_0 rows 0-7 (f[0], f[1]), M_1 rows 8-15 (f[2], f[3])
       d4 p0 = {0.0, 0.0, 0.0, 0.0}, p1 = {0.0, 0.0, 0.0, 0.0};
       p0 = mfma(e.v[0], f[0], p0);
