@@ -165,6 +165,10 @@ int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride);
  * it forms X = E (M_0 + M_1) and Y = (M_0 + M_1)^T X once instead of the
  * four products X_i = E M_i, Y_ab = M_a^T X_b (Y_ab is a block of Y).  Only
  * exact zeros are dropped, so energies are the same with it off (tested).
+ * The same pass stores F = A_0 + A_1 of every site (both copies), which
+ * holds both components exactly where their supports are disjoint, and
+ * marks each chain whose steps mostly have such structure; the I/Z energy
+ * chains read F (half the fragment bytes) and the rest keep the full sites.
  * The states of CNOT-entangled ansatze have this structure: in the 40-qubit
  * workload every bulk site's A_0 and A_1 occupy disjoint halves of the left
  * bond.  No reference counterpart (simulator_mps_dist.cu contracts dense
@@ -174,8 +178,9 @@ int gvmps_batch_set_sparsity(gvmps_batch *b, int on);
 /* FP64 flops the forest-chain launches of the batch's last energy call
  * executed, summed over its instances: per chain step the products the
  * kernel issues (full, row-split or merged, from the sites' zero structure
- * of that call; three real products per complex product on the complex
- * path).  For the roofline of the chain kernel; no reference counterpart. */
+ * and the chain marks of that call; three real products per complex product
+ * on the complex path).  For the roofline of the chain kernel; reads the
+ * marks back with a blocking copy.  No reference counterpart. */
 int gvmps_batch_chain_flops(gvmps_batch *b, double *flops);
 
 /* Launch configuration of the forest-chain kernels: wavefronts per workgroup
