@@ -197,9 +197,12 @@ __device__ __forceinline__ void rstep_children(const ProbDev &P, const InstDev &
   for (int c = 0; c < sd.nchild; ++c) {
     const ChildDev cd = uload(P.children + sd.child0 + c);
     if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
+    // ya +- yb as one FMA with a wave-uniform sign: the product is exact, so
+    // each entry is the sum or difference rounded once, as an add would be
+    const double sgn = cd.kind == CK_Z ? -1.0 : 1.0;
     REnv ce;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
+    for (int v = 0; v < 4; ++v) ce.v[v] = __builtin_fma(sgn, yb[v], ya[v]);
     if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
     if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
     const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
@@ -616,7 +619,7 @@ __device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, 
   const d4 ya = rprod_y(xk ? x1 : x0, f, 0);
   const d4 yb = rprod_y(xk ? x0 : x1, f, 1);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) e.v[v] = kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
+  for (int v = 0; v < 4; ++v) e.v[v] = __builtin_fma(kind == CK_Z ? -1.0 : 1.0, yb[v], ya[v]);  // ya +- yb
 }
 
 // Stores one step's site adjoint (m0, m1: Mbar_0, Mbar_1 in the D layout),
@@ -708,7 +711,7 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
         if (!IZ && (cd.kind == CK_X) != (pass == 1)) continue;
         REnv ce;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) ce.v[v] = cd.kind == CK_Z ? ya[v] - yb[v] : ya[v] + yb[v];
+        for (int v = 0; v < 4; ++v) ce.v[v] = __builtin_fma(cd.kind == CK_Z ? -1.0 : 1.0, yb[v], ya[v]);  // ya +- yb
         if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
         if (cd.gleaf >= 0) renv_store_sym(rslot(gl, cd.gleaf), lane, ce);
         if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
