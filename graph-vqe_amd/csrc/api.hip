@@ -301,6 +301,22 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
                     [&](const int2 &a) { return B->probs[B->inst[a.x].prob]->leaves[a.y].nterm; });
   B->leaf_items = upload(B->owned, leaf_items.data(), leaf_items.size());
   B->n_leaf_items = (int)leaf_items.size();
+  {
+    std::vector<LeafItem> desc(leaf_items.size());
+    for (size_t k = 0; k < leaf_items.size(); ++k) {
+      const InstDev &in = B->inst[leaf_items[k].x];
+      const Problem &P = *B->probs[in.prob];
+      const LeafDev &lf = P.leaves[leaf_items[k].y];
+      LeafItem &d = desc[k];
+      d.tl = pd[in.prob].term_list + lf.term0;
+      d.env = (in.leaf_off + leaf_items[k].y) * GV_ENV;
+      d.spill = in.spill_off;
+      d.part = in.part_off + (long long)P.chains.size() + leaf_items[k].y;
+      d.tv = in.term_off;
+      d.nterm = lf.nterm;
+    }
+    B->leaf_desc = upload(B->owned, desc.data(), desc.size());
+  }
   // chain work items per generation, instance-major, and the same items
   // longest chains first within each XCD's blocks
   for (int g = 0; g < B->max_gen; ++g) {
@@ -385,58 +401,9 @@ void check_status(Batch &B, hipStream_t stream) {
                                " is insufficient for zero-cutoff simulation: resolved singular values would be "
                                "discarded");  // dist_svd.cuh:277-279
     }
-    if (st[i] == ST_SCHED) {
-      // a grid barrier of the persistent forest launch expired: clear its
-      // words for the next call and discard this one's results
-      hip_check(hipMemsetAsync(B.psync, 0, sizeof(int) * (4 + 8 * (size_t)B.nphase), stream), "hipMemsetAsync");
-      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      throw std::runtime_error("persistent forest launch: a grid barrier waited longer than 0.2 s (not every "
-                               "workgroup was resident); results discarded");
-    }
     if (st[i] != ST_OK)
       throw std::runtime_error("Direct MPS SVD failed or produced nonfinite singular values; devInfo=0");
   }
-}
-
-// Phase table of the persistent forest launch: each generation's chain items
-// and the deferred leaves with their XCD shares (the ranges the generation
-// launches give each XCD), then the energy reduction.
-static void build_phases(Batch &B) {
-  auto shares = [](int n, int wpb, int *share) {
-    const int nb = (n + wpb - 1) / wpb, q8 = nb >> 3, r8 = nb & 7;
-    for (int x = 0; x < 8; ++x) share[x] = std::min(n, (x * q8 + std::min(x, r8)) * wpb);
-    share[8] = n;
-  };
-  std::vector<PhaseDev> ph;
-  for (int g = 0; g < B.max_gen; ++g) {
-    if (B.n_items[g] == 0) continue;
-    PhaseDev d{};
-    d.items = reinterpret_cast<const int *>(B.items_len[g]);
-    d.kind = 0;
-    d.count = B.n_items[g];
-    shares(d.count, B.wpb, d.share);
-    ph.push_back(d);
-  }
-  if (B.n_leaf_items > 0) {
-    PhaseDev d{};
-    d.items = reinterpret_cast<const int *>(B.leaf_items);
-    d.kind = 1;
-    d.count = B.n_leaf_items;
-    shares(d.count, 4, d.share);
-    ph.push_back(d);
-  }
-  PhaseDev r{};
-  r.kind = 2;
-  r.count = (int)B.inst.size();
-  ph.push_back(r);
-  if (!B.d_phases) {
-    // capacity for every generation, the leaves and the reduction
-    B.d_phases = upload<PhaseDev>(B.owned, nullptr, B.max_gen + 2);
-    B.psync = upload<int>(B.owned, nullptr, 4 + 8 * (size_t)(B.max_gen + 2));
-  }
-  hip_check(hipMemcpy(B.d_phases, ph.data(), ph.size() * sizeof(PhaseDev), hipMemcpyHostToDevice), "hipMemcpy");
-  B.nphase = (int)ph.size();
-  B.phases_dirty = false;
 }
 
 void launch_prepare(Batch &B, const double *params, hipStream_t stream) {
@@ -465,21 +432,6 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
       hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags),
                 "k_chain_flags launch");
     }
-    if (B.order == 2 && !terms) {
-      // one persistent launch: generations, leaves and reduction as phases
-      if (B.phases_dirty) build_phases(B);
-      int &grid = B.persist_grid[B.iz ? 1 : 0];
-      if (grid < 0) grid = persist_grid(B.iz, B.device);
-      if (grid <= 0) throw std::runtime_error("persistent forest launch: no resident workgroups");
-      B.kbegin(stream);
-      hip_check(kl_rforest_persist(grid, B.iz, stream, B.d_phases, B.nphase, B.psync, B.d_inst, ninst, B.d_probs,
-                                   B.rarena, B.n_site_total, B.leafs, B.spills, B.parts, B.energies, B.status),
-                "k_rforest_persist launch");
-      B.kend(stream);
-      B.mark(3, stream);
-      B.mark(4, stream);
-      return;
-    }
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
@@ -491,7 +443,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
                 "k_rchains launch");
       B.kend(stream);
     }
-    hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills,
+    hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_desc, B.leafs, B.spills,
                              terms ? B.terms : nullptr, B.parts),
               "k_rleaf_terms launch");
     B.mark(3, stream);
@@ -655,7 +607,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
                                 B.spills, B.parts, B.estore, B.gleafs, B.leafs),
                 "k_rchains_gfwd launch");
     }
-    hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_items, B.d_inst, B.d_probs, B.leafs, B.spills, nullptr,
+    hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_desc, B.leafs, B.spills, nullptr,
                              B.parts),
               "k_rleaf_terms launch");
     B.mark(2, stream);
@@ -980,8 +932,6 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
     Batch &B = *reinterpret_cast<Batch *>(b);
     if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
       throw std::runtime_error("waves per workgroup must be 1, 2 or 4");
-    if (order == 2)
-      throw std::runtime_error("order 2 (persistent forest launch) is disabled: its first GPU run did not finish");
     if (order != 0 && order != 1) throw std::runtime_error("order must be 0 (instance-major) or 1 (longest first)");
     if (waves_per_group != B.wpb) {  // the XCD ranges depend on the workgroup size
       hip_check(hipSetDevice(B.device), "hipSetDevice");
@@ -1001,7 +951,6 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
     }
     B.wpb = waves_per_group;
     B.order = order;
-    B.phases_dirty = true;
   })
 }
 

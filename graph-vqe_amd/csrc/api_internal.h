@@ -37,6 +37,7 @@ struct Batch {
   int max_terms = 0;
   double2 *leafs = nullptr;       // deferred term-end leaf environments
   int2 *leaf_items = nullptr;     // (instance, leaf) work items
+  LeafItem *leaf_desc = nullptr;  // the same items as descriptors (real leaf kernels)
   int n_leaf_items = 0;
   std::vector<int2 *> items;      // per generation, instance-major
   std::vector<int2 *> items_len;  // per generation, longest chains first within each XCD's blocks (for wpb)
@@ -63,12 +64,6 @@ struct Batch {
   double *rfarena = nullptr;  // merged fragments A_0 + A_1 of each slot, direct then transposed (k_site_masks)
   int *cflags = nullptr;      // per (instance, chain): its steps run on the merged fragments (k_chain_flags)
   bool use_smask = true;      // chain steps skip exact-zero products (gvmps_batch_set_sparsity)
-  // persistent forest launch (order 2): phase table, barrier and pop words
-  PhaseDev *d_phases = nullptr;
-  int nphase = 0;
-  int *psync = nullptr;   // 4 barrier words + 8 pop counters per phase, zero between launches
-  bool phases_dirty = true;
-  int persist_grid[2] = {-1, -1};  // resident workgroups of k_rforest_persist / _iz
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   const int2 *ggen_items(int g) const { return order ? gitems_len[g] : gitems[g]; }
   // captured evaluate + sample sequence
@@ -145,13 +140,8 @@ hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *i
 hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask, double *farena);
 hipError_t kl_chain_flags(int ninst, int max_chains, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                           const unsigned *smask, int *cflags);
-hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                          const double2 *leafs, const double2 *spills, double2 *terms, double *parts);
-int persist_grid(bool iz, int device);
-hipError_t kl_rforest_persist(int grid, bool iz, hipStream_t s, const PhaseDev *phases, int nphase, int *sync,
-                              const InstDev *insts, int ninst, const ProbDev *probs, const double *rarena,
-                              long long nsite, double2 *leafs, double2 *spills, double *parts, double *energies,
-                              int *status);
+hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const LeafItem *items, const double2 *leafs,
+                          const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_rchains_gfwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
                            double2 *estore, double2 *gleafs, double2 *leafs);

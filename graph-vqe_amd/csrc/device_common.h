@@ -394,43 +394,4 @@ __device__ __forceinline__ void reduce_energy_block(const gv::InstDev &inst, con
   __syncthreads();  // ssum / scomp are reused by the workgroup's next instance
 }
 
-// ------------------------------------------------ grid barrier
-// Barrier across every workgroup of a persistent launch, all of them
-// resident (the launcher sizes the grid from the kernel's occupancy).
-// sync[0] counts arrivals, sync[1] is the epoch, sync[2] the poison flag.
-// Thread 0 of each workgroup reads the epoch, then arrives; the last to
-// arrive zeroes the count and the finished phase's pop counters (`ctr`, 8
-// words, no longer touched by anyone) and advances the epoch, which the
-// others wait for.  All threads fence at agent scope before and after, so a
-// phase's stores are visible to the next phase's loads on every XCD.  The
-// wait is bounded (GV_BARRIER_TICKS of the 100 MHz wall clock): on expiry
-// the launch is poisoned, every later barrier returns at once, and each
-// instance's status reads ST_SCHED, so the call fails instead of hanging.
-#define GV_BARRIER_TICKS 20000000ull  // 0.2 s
-__device__ __forceinline__ void grid_barrier(int *sync, int *ctr, int *status, int ninst) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0 && __hip_atomic_load(sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-    const int epoch = __hip_atomic_load(sync + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const int arrived = __hip_atomic_fetch_add(sync, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == (int)gridDim.x - 1) {
-      for (int x = 0; x < 8; ++x) __hip_atomic_store(ctr + x, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(sync, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(sync + 1, epoch + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const unsigned long long t0 = wall_clock64();
-      while (__hip_atomic_load(sync + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) {
-        if (__hip_atomic_load(sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (wall_clock64() - t0 > GV_BARRIER_TICKS) {
-          __hip_atomic_store(sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          for (int i = 0; i < ninst; ++i) status[i] = gv::ST_SCHED;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-  }
-  __syncthreads();
-  __threadfence();
-}
 

@@ -104,11 +104,28 @@ struct LeafDev {
   int term0, nterm;
 };
 
-// A term ending at a prefix-forest node: its index and the slot of its
-// suffix-forest environment (-1: identity right environment Rt[site+1]).
+// A term ending at a prefix-forest node: its index, the slot of its
+// suffix-forest environment (-1: identity right environment Rt[site+1]) and
+// Re c_k, so a term end's one scalar load brings its coefficient too.
 struct TermEnd {
   int term;
   int rslot;
+  double cre;
+};
+
+// One deferred leaf of a batch, with everything its wavefront needs in one
+// scalar load (the real leaf kernels): its term ends, the offsets (in 16-byte
+// units) of its leaf environment in the leaf arena and of its instance's
+// suffix-forest slots in the spill arena, and where its partial energy and
+// (per-term runs) its term values go.
+struct LeafItem {
+  const TermEnd* tl;
+  long long env;
+  long long spill;
+  long long part;
+  long long tv;
+  int nterm;
+  int pad;
 };
 
 // A term whose suffix-forest environment sits in a store slot; its prefix
@@ -184,19 +201,9 @@ struct InstDev {
   long long leaf_off;  // deferred term-end leaf environments (4 KiB)
 };
 
-// Status codes written by kernels (per instance).
-enum Status : int { ST_OK = 0, ST_GUARD = 2, ST_NONFINITE = 1, ST_SCHED = 3 };
 
-// One phase of the persistent forest launch (k_rforest_persist): a forest
-// generation's chains, the deferred leaves, or the energy reduction.  Chain
-// and leaf items are stored as for the generation launches, each XCD's share
-// of them contiguous: share[x] .. share[x + 1].
-struct PhaseDev {
-  const int* items;    // (instance, item) pairs
-  int kind;            // 0 chains, 1 deferred leaves, 2 energy reduction (a workgroup per instance)
-  int count;
-  int share[9];
-  int pad;
-};
+// Status codes written by kernels (per instance).
+enum Status : int { ST_OK = 0, ST_GUARD = 2, ST_NONFINITE = 1 };
+
 
 }  // namespace gv

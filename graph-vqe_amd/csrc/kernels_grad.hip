@@ -195,7 +195,7 @@ k_chains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restr
         double sr = 0.0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) sr += ce.re[v] * r.re[v] - ce.im[v] * r.im[v];
-        acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c_k) Re<P_k>
+        acc += te.cre * sr;  // Re(c_k) Re<P_k>
       }
       if (cd.action == -2) cont = ce;
       else if (cd.action >= 0) env_store_herm(spill + (size_t)cd.action * GV_ENV, lane, ce);
@@ -364,7 +364,7 @@ k_chains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restri
       // term ends (prefix forest): Cbar += Re(c_k) conj(Rt_k)
       for (int t = 0; t < cd.nterm; ++t) {
         const TermEnd te = uload(P.term_list + cd.term0 + t);
-        const double ck = uload(P.coeffs + 2 * te.term);
+        const double ck = te.cre;
         if (te.rslot >= 0) {
           Env r;
           env_load_herm(spill + (size_t)te.rslot * GV_ENV, lane, r);

@@ -19,6 +19,7 @@
 // those of kernels_energy.hip, so k_reduce_energy / k_reduce combine the
 // results unchanged.
 #include "device_common.h"
+#include "site_pack.h"
 
 using namespace gv;
 
@@ -124,20 +125,8 @@ __device__ __forceinline__ d4 rprod_y(const d4 &x, const RFrag &f, int j) {
   return y;
 }
 
-// Zero structure of a site, per (instance, site, copy), from k_site_masks:
-//   bits 0-3 / 4-7   row groups t (rows 4t..4t+3) of M_0 / M_1 with a nonzero
-//   bit 8            the column supports of M_0 and M_1 are disjoint
-//   bits 16-31       the columns of M_1
-// Both products of a step contract over the rows of M_i in k-steps of four
-// rows, so a k-step over an all-zero row group only adds exact zeros and can
-// be skipped; the chain steps do so for the row split of bulk sites (M_0 in
-// rows 0-7, M_1 in rows 8-15: groups 0-1 and 2-3), in straight-line code (a
-// branch per k-step would break the MFMA issue).  With disjoint columns, X = E (M_0 + M_1) holds X_0 = E M_0 in
-// the columns of M_0 and X_1 in those of M_1, and Y = (M_0 + M_1)^T X holds
-// M_a^T X_b in the block (columns of M_a) x (columns of M_b): one product
-// pair gives Y00, Y11, Y01 and Y10, each entry from the same products in the
-// same order as the separate products (the rest being exact zeros).
-#define GV_SM_DISJOINT 0x100u
+// Zero structure of the sites (site_pack.h): the mask words select, per
+// step, the merged and row-split products below.
 // Products over the k-steps in the compile-time set G (row groups of M_i).
 template <unsigned G>
 __device__ __forceinline__ d4 rprod_x_g(const REnv &e, const RFrag &f, int i) {
@@ -219,7 +208,7 @@ __device__ __forceinline__ void rstep_children(const ProbDev &P, const InstDev &
         sr = wave_sum(sr);
         if (lane == 0) tv[te.term] = make_double2(sr, 0.0);
       } else {
-        acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c <P>), <P> real
+        acc += te.cre * sr;  // Re(c <P>), <P> real
       }
     }
     if (cd.action == -2) cont = ce;
@@ -339,8 +328,6 @@ __device__ __forceinline__ void rchains_body(const int2 *__restrict__ items, int
 // loads 2 KiB instead of 4 and keeps 4 doubles per lane per fragment instead
 // of 8; the few steps whose site has no such structure (bonds below 16 at
 // the chain ends) load the full slot when they run.
-#define GV_FSLOT (GV_CHI * GV_CHI)
-
 __device__ __forceinline__ void ffrag_load(const double *slot, int lane, double (&f)[4]) {
   const int j = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -447,11 +434,6 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
   }
 }
 
-// A site step has structure when its components are column-disjoint or row
-// split (the two cases the merged fragments serve).
-__device__ __forceinline__ bool site_structured(unsigned mw) {
-  return (mw & GV_SM_DISJOINT) != 0u || (mw & 0xffu) == 0xc3u;
-}
 
 // Chains with at most a quarter of their steps on sites without structure run
 // over the merged fragments (those steps load their full slots when they run);
@@ -482,122 +464,79 @@ __device__ __forceinline__ void rchains_merged_body(const int2 *__restrict__ ite
                              parts);
 }
 
+// Deferred term ends contract two symmetric environments, both stored as
+// packed upper triangles (136 doubles): <L, R> = sum over the triangle of
+// w L R with w = 1 on the diagonal and 2 off it.  Lane l reads packed
+// elements 2l, 2l+1 (one 16-byte load) and lanes 0-3 also 128 + 2l, 129 + 2l,
+// so each environment is two contiguous loads per lane instead of four
+// gathers of the mirrored square.  The weights are folded into L once.
+__device__ __forceinline__ bool sym_diag(int p) {
+  // diagonal positions of the packed triangle: a (33 - a) / 2, a = 0..15
+  return p == 0 || p == 16 || p == 31 || p == 45 || p == 58 || p == 70 || p == 81 || p == 91 || p == 100 ||
+         p == 108 || p == 115 || p == 121 || p == 126 || p == 130 || p == 133 || p == 135;
+}
+
+struct RPacked {
+  double2 lo, hi;  // packed elements 2l, 2l+1 and (lanes 0-3) 128 + 2l, 129 + 2l
+};
+
+__device__ __forceinline__ void rpacked_load(const double *p, int lane, RPacked &e) {
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  e.lo = q[lane];
+  e.hi = lane < 4 ? q[64 + lane] : make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ void rpacked_unit(int lane, RPacked &e) {
+  e.lo = make_double2(lane == 0 ? 1.0 : 0.0, 0.0);
+  e.hi = make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ void rpacked_weight(int lane, RPacked &e) {
+  if (!sym_diag(2 * lane)) e.lo.x *= 2.0;
+  if (!sym_diag(2 * lane + 1)) e.lo.y *= 2.0;
+  if (!sym_diag(128 + 2 * lane)) e.hi.x *= 2.0;
+  if (!sym_diag(129 + 2 * lane)) e.hi.y *= 2.0;
+}
+
 template <bool TERMS>
-__device__ __forceinline__ void rleaf_item(const int2 it, const int lane, const InstDev *__restrict__ insts,
-                                           const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs,
+__device__ __forceinline__ void rleaf_item(const LeafItem &li, const int lane, const double2 *__restrict__ leafs,
                                            const double2 *__restrict__ spills, double2 *__restrict__ terms,
                                            double *__restrict__ parts) {
-  const InstDev inst = uload(insts + it.x);
-  const ProbDev P = uload(probs + inst.prob);
-  const LeafDev lf = uload(P.leaves + it.y);
-  const double2 *spill = spills + inst.spill_off;
-  REnv e;
-  renv_load_sym(rslot(leafs, inst.leaf_off + it.y), lane, e);
+  const double2 *spill = spills + li.spill;
+  RPacked e;
+  rpacked_load((const double *)(leafs + li.env), lane, e);
+  rpacked_weight(lane, e);
   double acc = 0.0;
 #pragma unroll 4
-  for (int t = 0; t < lf.nterm; ++t) {
-    const TermEnd te = uload(P.term_list + lf.term0 + t);
-    REnv r;
-    if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
-    else renv_unit(lane, r);
-    double sr = 0.0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) sr += e.v[v] * r.v[v];
+  for (int t = 0; t < li.nterm; ++t) {
+    const TermEnd te = uload(li.tl + t);
+    RPacked r;
+    if (te.rslot >= 0) rpacked_load(rslot(spill, te.rslot), lane, r);
+    else rpacked_unit(lane, r);
+    double sr = e.lo.x * r.lo.x;
+    sr = __builtin_fma(e.lo.y, r.lo.y, sr);
+    sr = __builtin_fma(e.hi.x, r.hi.x, sr);
+    sr = __builtin_fma(e.hi.y, r.hi.y, sr);
     if (TERMS) {
       sr = wave_sum(sr);
-      if (lane == 0) terms[inst.term_off + te.term] = make_double2(sr, 0.0);
+      if (lane == 0) terms[li.tv + te.term] = make_double2(sr, 0.0);
     } else {
-      acc += uload(P.coeffs + 2 * te.term) * sr;
+      acc = __builtin_fma(te.cre, sr, acc);
     }
   }
   if (!TERMS) {
     acc = wave_sum(acc);
-    if (lane == 0) parts[inst.part_off + P.nchains + it.y] = acc;
+    if (lane == 0) parts[li.part] = acc;
   }
 }
 
 template <bool TERMS>
-__device__ __forceinline__ void rleaf_body(const int2 *__restrict__ items, int nitems,
-                                           const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
+__device__ __forceinline__ void rleaf_body(const LeafItem *__restrict__ items, int nitems,
                                            const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
                                            double2 *__restrict__ terms, double *__restrict__ parts) {
   const int w = xcd_item();
   if (w >= nitems) return;
-  rleaf_item<TERMS>(uload(items + w), threadIdx.x & 63, insts, probs, leafs, spills, terms, parts);
-}
-
-// The forest pass of an energy call in one persistent launch: every
-// generation's chains, the deferred leaves and the energy reduction as
-// phases of one kernel, separated by grid barriers (grid_barrier), instead of
-// one launch each.  Each wave takes chain or leaf items from its XCD's share
-// of the phase (an atomic counter per share, then the other shares), so
-// instances stay on the XCD, and in the L2, that holds their sites, as in
-// the generation launches; the per-item code is theirs, so energies are
-// bitwise the same.  The grid holds only resident workgroups, sized by the
-// launcher from the kernel's occupancy.
-// The light phases of the persistent launch stay out of line, so that the
-// chain item inlined in its loop keeps the registers of the generation
-// launches' kernels.
-__device__ __attribute__((noinline)) void rleaf_item_call(const int2 it, const int lane,
-                                                          const InstDev *__restrict__ insts,
-                                                          const ProbDev *__restrict__ probs,
-                                                          const double2 *__restrict__ leafs,
-                                                          const double2 *__restrict__ spills,
-                                                          double *__restrict__ parts) {
-  rleaf_item<false>(it, lane, insts, probs, leafs, spills, nullptr, parts);
-}
-
-__device__ __attribute__((noinline)) void reduce_phase_call(const InstDev *__restrict__ insts, int ninst,
-                                                            const ProbDev *__restrict__ probs,
-                                                            const double *__restrict__ parts,
-                                                            double *__restrict__ energies, double *ssum,
-                                                            double *scomp) {
-  for (int i = blockIdx.x; i < ninst; i += gridDim.x) {
-    const InstDev inst = insts[i];
-    reduce_energy_block(inst, probs[inst.prob], parts, energies, 1, ssum, scomp);
-  }
-}
-
-template <bool IZ>
-__device__ __forceinline__ void rforest_persist_body(const PhaseDev *__restrict__ phases, int nphase,
-                                                     int *__restrict__ sync, const InstDev *__restrict__ insts,
-                                                     int ninst, const ProbDev *__restrict__ probs,
-                                                     const double *__restrict__ rarena,
-                                                     const double *__restrict__ rarenaT, double2 *__restrict__ leafs,
-                                                     double2 *__restrict__ spills, double *__restrict__ parts,
-                                                     double *__restrict__ energies, int *__restrict__ status) {
-  __shared__ double ssum[256], scomp[256];
-  const int lane = threadIdx.x & 63;
-  const int xcd = blockIdx.x & 7;
-#pragma nounroll
-  for (int ph = 0; ph < nphase; ++ph) {
-    // the phase's fields one word at a time (a copy of the whole struct, indexed
-    // by share below, would live in scratch)
-    const int kind = uload(&phases[ph].kind);
-    const int2 *items = reinterpret_cast<const int2 *>(uload(&phases[ph].items));
-    int *ctr = sync + 4 + 8 * ph;
-    if (kind == 2) {
-      reduce_phase_call(insts, ninst, probs, parts, energies, ssum, scomp);
-    } else {
-#pragma nounroll
-      for (int s = 0; s < 8; ++s) {
-        const int x = (xcd + s) & 7;
-        const int b0 = uload(&phases[ph].share[x]), n = uload(&phases[ph].share[x + 1]) - b0;
-        for (;;) {
-          int k = 0;
-          if (lane == 0) k = atomicAdd(ctr + x, 1);
-          k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
-          if (k >= n) break;
-          const int2 it = uload(items + b0 + k);
-          if (kind == 0)
-            rchain_item<false, IZ>(it, lane, insts, probs, rarena, rarenaT, nullptr, leafs, spills, nullptr, parts);
-          else
-            rleaf_item_call(it, lane, insts, probs, leafs, spills, parts);
-        }
-      }
-    }
-    if (ph + 1 < nphase) grid_barrier(sync, ctr, status, ninst);
-  }
+  rleaf_item<TERMS>(uload(items + w), threadIdx.x & 63, leafs, spills, terms, parts);
 }
 
 __device__ __forceinline__ int sidx(int l, int i, int r) { return (l * 2 + i) * 16 + r; }
@@ -724,7 +663,7 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
           double sr = 0.0;
 #pragma unroll
           for (int v = 0; v < 4; ++v) sr += ce.v[v] * r.v[v];
-          acc += uload(P.coeffs + 2 * te.term) * sr;  // Re(c_k) <P_k>
+          acc += te.cre * sr;  // Re(c_k) <P_k>
         }
         if (cd.action == -2) cont = ce;
         else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
@@ -824,7 +763,7 @@ __device__ __forceinline__ void rbwd_body(const int2 *__restrict__ items, int ni
       // term ends (prefix forest): Cbar += Re(c_k) Rt_k
       for (int t = 0; t < cd.nterm; ++t) {
         const TermEnd te = uload(P.term_list + cd.term0 + t);
-        const double ck = uload(P.coeffs + 2 * te.term);
+        const double ck = te.cre;
         if (te.rslot >= 0) {
           REnv r;
           renv_load_sym(rslot(spill, te.rslot), lane, r);
@@ -986,9 +925,7 @@ k_pack_real(const double2 *__restrict__ arena, double *__restrict__ rarena, doub
             long long n) {
   for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
     rarena[e] = arena[e].x;
-    const long long base = e - (e % GV_SLOT);
-    const int k = (int)(e % GV_SLOT), row = k >> 5, i = (k >> 4) & 1, j = k & 15;  // A^T[row][i][j] = A[j][i][row]
-    rarenaT[e] = arena[base + (j * 2 + i) * 16 + row].x;
+    rarenaT[e] = arena[pack_transposed_src(e)].x;
   }
 }
 
@@ -1046,38 +983,16 @@ k_rchains_energy_izm(const int2 *__restrict__ items, int nitems, const InstDev *
                              smask, cflags, leafs, spills, nullptr, parts);
 }
 
-extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
-k_rforest_persist(const PhaseDev *__restrict__ phases, int nphase, int *__restrict__ sync,
-                  const InstDev *__restrict__ insts, int ninst, const ProbDev *__restrict__ probs,
-                  const double *__restrict__ rarena, long long nsite, double2 *__restrict__ leafs,
-                  double2 *__restrict__ spills, double *__restrict__ parts, double *__restrict__ energies,
-                  int *__restrict__ status) {
-  rforest_persist_body<false>(phases, nphase, sync, insts, ninst, probs, rarena, rarena + nsite, leafs, spills, parts,
-                              energies, status);
-}
-
-extern "C" __global__ void __launch_bounds__(256) GV_IZ_WAVES
-k_rforest_persist_iz(const PhaseDev *__restrict__ phases, int nphase, int *__restrict__ sync,
-                     const InstDev *__restrict__ insts, int ninst, const ProbDev *__restrict__ probs,
-                     const double *__restrict__ rarena, long long nsite, double2 *__restrict__ leafs,
-                     double2 *__restrict__ spills, double *__restrict__ parts, double *__restrict__ energies,
-                     int *__restrict__ status) {
-  rforest_persist_body<true>(phases, nphase, sync, insts, ninst, probs, rarena, rarena + nsite, leafs, spills, parts,
-                             energies, status);
+extern "C" __global__ void __launch_bounds__(256)
+k_rleaf_terms(const LeafItem *__restrict__ items, int nitems, const double2 *__restrict__ leafs,
+              const double2 *__restrict__ spills, double2 *__restrict__ terms) {
+  rleaf_body<true>(items, nitems, leafs, spills, terms, nullptr);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
-k_rleaf_terms(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-              const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
-              double2 *__restrict__ terms) {
-  rleaf_body<true>(items, nitems, insts, probs, leafs, spills, terms, nullptr);
-}
-
-extern "C" __global__ void __launch_bounds__(256)
-k_rleaf_energy(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
-               const ProbDev *__restrict__ probs, const double2 *__restrict__ leafs, const double2 *__restrict__ spills,
-               double *__restrict__ parts) {
-  rleaf_body<false>(items, nitems, insts, probs, leafs, spills, nullptr, parts);
+k_rleaf_energy(const LeafItem *__restrict__ items, int nitems, const double2 *__restrict__ leafs,
+               const double2 *__restrict__ spills, double *__restrict__ parts) {
+  rleaf_body<false>(items, nitems, leafs, spills, nullptr, parts);
 }
 
 namespace gv {
@@ -1122,46 +1037,10 @@ k_site_masks(long long nslots, const double *__restrict__ rarena, unsigned *__re
              double *__restrict__ farena) {
   const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= nslots) return;
-  const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
-  const double *A = rarena + w * GV_SLOT;
-  unsigned rows[2] = {0u, 0u}, cols[2] = {0u, 0u};  // supports of A_i: rows l, columns r
-  double f[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const double a = A[((g + 4 * t) * 2 + i) * 16 + j];
-      f[t] += a;
-      const unsigned long long b = __ballot(a != 0.0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if ((b >> (16 * q)) & 0xffffull) rows[i] |= 1u << (q + 4 * t);
-      cols[i] |= (unsigned)((b | (b >> 16) | (b >> 32) | (b >> 48)) & 0xffffull);
-    }
-  // merged fragments F = A_0 + A_1 (direct copy) and its transpose
-  // (transposed copy), row-major; meaningful where the supports are disjoint
-  double *fd = farena + w * GV_FSLOT, *ft = farena + (nslots + w) * GV_FSLOT;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    fd[(g + 4 * t) * 16 + j] = f[t];
-    ft[j * 16 + g + 4 * t] = f[t];
-  }
-  auto groups = [](unsigned supp) {
-    unsigned m = 0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-      if ((supp >> (4 * t)) & 15u) m |= 1u << t;
-    return m;
-  };
-  // direct copy: rows of M_i are rows of A_i, columns are columns; transposed: swapped
-  const unsigned direct = groups(rows[0]) | (groups(rows[1]) << 4) |
-                          ((cols[0] & cols[1]) == 0u ? GV_SM_DISJOINT : 0u) | (cols[1] << 16);
-  const unsigned transposed = groups(cols[0]) | (groups(cols[1]) << 4) |
-                              ((rows[0] & rows[1]) == 0u ? GV_SM_DISJOINT : 0u) | (rows[1] << 16);
-  if (lane == 0) {
-    smask[2 * w] = direct;
-    smask[2 * w + 1] = transposed;
-  }
+  const int lane = threadIdx.x & 63;
+  double a[2][4];
+  site_slot_load(rarena + w * GV_SLOT, lane, a);
+  site_mask_slot(a, farena + w * GV_FSLOT, farena + (nslots + w) * GV_FSLOT, smask + 2 * w, lane);
 }
 
 hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask, double *farena) {
@@ -1180,22 +1059,7 @@ k_chain_flags(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
   const ProbDev P = uload(probs + inst.prob);
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   if (c >= P.nchains) return;
-  const ChainDev ch = P.chains[c];
-  const unsigned *sm = smask + (inst.site_off / GV_SLOT) * 2 + (ch.dir != 0 ? 1 : 0);
-  // eight steps per round: their descriptor loads, then their mask loads,
-  // are issued together instead of one dependent pair per step
-  int plain = 0;
-  for (int s0 = 0; s0 < ch.nsteps; s0 += 8) {
-    int site[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) site[q] = s0 + q < ch.nsteps ? P.steps[ch.step0 + s0 + q].site : -1;
-    unsigned mw[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) mw[q] = site[q] >= 0 ? sm[2 * site[q]] : GV_SM_DISJOINT;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) plain += site_structured(mw[q]) ? 0 : 1;
-  }
-  cflags[inst.part_off + c] = 4 * plain <= ch.nsteps ? 1 : 0;
+  cflags[inst.part_off + c] = chain_flag(P, c, smask + (inst.site_off / GV_SLOT) * 2);
 }
 
 hipError_t kl_chain_flags(int ninst, int max_chains, hipStream_t s, const InstDev *insts, const ProbDev *probs,
@@ -1228,36 +1092,13 @@ hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *i
   return hipGetLastError();
 }
 
-// Resident workgroups of the persistent forest kernel: its occupancy per CU
-// times the CUs, a multiple of 8 (workgroup b runs on XCD b % 8).
-int persist_grid(bool iz, int device) {
-  int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(iz ? k_rforest_persist_iz
-                                                                                              : k_rforest_persist),
-                                                   256, 0) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
-    return 0;
-  return (per_cu * cus) & ~7;
-}
-
-hipError_t kl_rforest_persist(int grid, bool iz, hipStream_t s, const PhaseDev *phases, int nphase, int *sync,
-                              const InstDev *insts, int ninst, const ProbDev *probs, const double *rarena,
-                              long long nsite, double2 *leafs, double2 *spills, double *parts, double *energies,
-                              int *status) {
-  hipLaunchKernelGGL(iz ? k_rforest_persist_iz : k_rforest_persist, dim3(grid), dim3(256), 0, s, phases, nphase, sync,
-                     insts, ninst, probs, rarena, nsite, leafs, spills, parts, energies, status);
-  return hipGetLastError();
-}
-
-hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const int2 *items, const InstDev *insts, const ProbDev *probs,
-                          const double2 *leafs, const double2 *spills, double2 *terms, double *parts) {
+hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const LeafItem *items, const double2 *leafs,
+                          const double2 *spills, double2 *terms, double *parts) {
   if (nitems == 0) return hipSuccess;
   if (terms)
-    hipLaunchKernelGGL(k_rleaf_terms, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, leafs,
-                       spills, terms);
+    hipLaunchKernelGGL(k_rleaf_terms, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, leafs, spills, terms);
   else
-    hipLaunchKernelGGL(k_rleaf_energy, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, insts, probs, leafs,
-                       spills, parts);
+    hipLaunchKernelGGL(k_rleaf_energy, dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, leafs, spills, parts);
   return hipGetLastError();
 }
 

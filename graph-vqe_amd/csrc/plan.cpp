@@ -450,7 +450,7 @@ void build_chains(Problem &P) {
   for (size_t i = 0; i < paths.size(); ++i) {
     const TermPath &t = paths[i];
     const int ln = L.walk(P.site_words[t.k], 0, t.c, +1);
-    L.nodes[ln].ends.push_back(TermEnd{t.k, right_leaf[i]});
+    L.nodes[ln].ends.push_back(TermEnd{t.k, right_leaf[i], P.coeffs[2 * (size_t)t.k]});
   }
   P.n_nodes = (int)L.nodes.size();
   P.n_right_nodes = (int)R.nodes.size();

@@ -157,9 +157,9 @@ class Batch:
     def set_launch(self, waves_per_group=4, order=1):
         """Chain-kernel launch shape (results do not depend on it): waves per
         workgroup, and item order 0 (instance-major), 1 (longest chains
-        first within each XCD's share of the instance-major items); order 2
-        (the forest pass in one persistent launch) is rejected by the
-        library."""
+        first within each XCD's share of the instance-major items).  The
+        library has no persistent forest launch; other orders are
+        rejected."""
         check(lib().gvmps_batch_set_launch(self._h, int(waves_per_group), int(order)))
         self.launch = (int(waves_per_group), int(order))
 

@@ -257,7 +257,7 @@ def test_spsa_40q_config_parity_and_batch_invariance(cuda):
     assert np.all(np.abs((tv * c).sum(axis=1).real - e_terms) <= 1e-10 * np.abs(e_terms))
 
 
-@pytest.mark.skip(reason="launch order 2 (the forest pass as one persistent launch: generations, deferred leaves and the energy reduction as phases behind grid barriers bounded by 0.2 s of wall clock) did not return within the test's 120 s on its first GPU call (40-qubit batch of 256, 4 waves per group); reading the kernel, its barrier and pop loop and their ISA showed no cause, so set_launch rejects order 2")
+@pytest.mark.skip(reason="launch order 2 (the forest pass as one persistent launch behind grid barriers) did not return within the test's 120 s on its first GPU call (40-qubit batch of 256, 4 waves per group); no cause was found by reading the kernel, so the kernel was taken out of the library and set_launch rejects order 2")
 def test_persistent_forest_launch_bitwise(cuda):
     """Launch order 2 (the forest pass of an energy call in one persistent
     launch: every generation's chains, the deferred leaves and the energy

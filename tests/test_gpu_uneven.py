@@ -102,7 +102,7 @@ def test_graph_replay_matches_eager(uneven):
     assert torch.equal(e, e2) and torch.equal(b, b2)
 
 
-@pytest.mark.skip(reason="launch order 2 (the forest pass as one persistent launch: generations, deferred leaves and the energy reduction as phases behind grid barriers bounded by 0.2 s of wall clock) did not return within the test's 120 s on its first GPU call (40-qubit batch of 256, 4 waves per group); reading the kernel, its barrier and pop loop and their ISA showed no cause, so set_launch rejects order 2")
+@pytest.mark.skip(reason="launch order 2 (the forest pass as one persistent launch behind grid barriers) did not return within the test's 120 s on its first GPU call (40-qubit batch of 256, 4 waves per group); no cause was found by reading the kernel, so the kernel was taken out of the library and set_launch rejects order 2")
 def test_persistent_launch_evaluate_and_replay(uneven):
     """The forest pass in one persistent launch (order 2) inside evaluate and
     inside the captured graph: energies and bitstrings bitwise equal to the
