@@ -183,7 +183,13 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     d.fixes = upload(B->owned, P.fixes.data(), P.fixes.size());
     d.chains = upload(B->owned, P.chains.data(), P.chains.size());
     d.steps = upload(B->owned, P.steps.data(), P.steps.size());
-    d.children = upload(B->owned, P.children.data(), P.children.size());
+    {
+      // one zero record past the end: chain loops fetch a step's first two
+      // child records unconditionally, one step ahead
+      std::vector<ChildDev> ch(P.children);
+      ch.push_back(ChildDev{});
+      d.children = upload(B->owned, ch.data(), ch.size());
+    }
     d.term_list = upload(B->owned, P.term_list.data(), P.term_list.size());
     d.coeffs = upload(B->owned, P.coeffs.data(), P.coeffs.size());
     std::vector<int> act(P.nterms, 0);

@@ -216,6 +216,45 @@ __device__ __forceinline__ void rstep_children(const ProbDev &P, const InstDev &
   }
 }
 
+// The children of an I/Z step (at most two: I and Z) from records the chain
+// loop fetched one step ahead, so no descriptor load waits here.
+template <bool TERMS>
+__device__ __forceinline__ void rstep_children_iz(const ProbDev &P, const InstDev &inst, int nchild,
+                                                  const ChildDev (&cds)[2], const d4 &ya, const d4 &yb, int lane,
+                                                  double2 *__restrict__ spill, double2 *__restrict__ leafs,
+                                                  double2 *__restrict__ tv, double &acc, REnv &cont) {
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (c >= nchild) break;
+    const ChildDev &cd = cds[c];
+    const double sgn = cd.kind == CK_Z ? -1.0 : 1.0;
+    REnv ce;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ce.v[v] = __builtin_fma(sgn, yb[v], ya[v]);
+    if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
+    if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
+    const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
+#pragma unroll(GV_IZ_TERM_UNROLL)
+    for (int t = 0; t < nterm_inline; ++t) {
+      const TermEnd te = uload(P.term_list + cd.term0 + t);
+      REnv r;
+      if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
+      else renv_unit(lane, r);  // term ends at site n-1
+      double sr = 0.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sr += ce.v[v] * r.v[v];
+      if (TERMS) {
+        sr = wave_sum(sr);
+        if (lane == 0) tv[te.term] = make_double2(sr, 0.0);
+      } else {
+        acc += te.cre * sr;
+      }
+    }
+    if (cd.action == -2) cont = ce;
+    else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
+  }
+}
+
 // One generation of prefix/suffix chains in real arithmetic; the same work
 // items and slot numbering as chains_body in kernels_energy.hip.  IZ: every
 // step of the forest has I/Z children only (Hamiltonians without X words), so
@@ -359,14 +398,30 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
   else renv_unit(lane, e);
   double acc = 0.0;
 
-  StepDev sd0 = uload(P.steps + ch.step0);
+  // Scalar pipeline: the descriptors a step needs (its StepDev, mask word and
+  // child records) are fetched one step ahead, beside the fragment prefetch,
+  // so the step's only scalar waits are for loads issued a whole step before.
+  auto children_of = [&](const StepDev &d, ChildDev (&c)[2]) {
+    // unconditional (the batch pads the child table with one record), so no
+    // branch or copy waits for them before they are used
+    c[0] = uload(P.children + d.child0);
+    c[1] = uload(P.children + d.child0 + 1);
+  };
+  StepDev sd = uload(P.steps + ch.step0);
+  StepDev nsd = uload(P.steps + ch.step0 + min(1, ch.nsteps - 1));
+  unsigned mw = uload(sm + 2 * sd.site);
+  ChildDev cds[2];
+  children_of(sd, cds);
   double fa[4], fb[4];
-  ffrag_load(fsites + (size_t)sd0.site * GV_FSLOT, lane, fa);
-  // unconditional prefetch of the next step's merged fragment, as in rchain_item
-  auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const double(&f)[4], double(&nf)[4]) {
-    nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));
+  ffrag_load(fsites + (size_t)sd.site * GV_FSLOT, lane, fa);
+  auto run_step = [&](int st, const double(&f)[4], double(&nf)[4]) {
+    // next step: fragment (unconditional, as in rchain_item), mask, children;
+    // the step after: its descriptor
     ffrag_load(fsites + (size_t)nsd.site * GV_FSLOT, lane, nf);
-    const unsigned mw = uload(sm + 2 * sd.site);
+    const unsigned nmw = uload(sm + 2 * nsd.site);
+    ChildDev ncds[2];
+    children_of(nsd, ncds);
+    const StepDev nnsd = uload(P.steps + ch.step0 + min(st + 2, ch.nsteps - 1));
     d4 y0, y1;
     if (mw & GV_SM_DISJOINT) {
       d4 x = {0.0, 0.0, 0.0, 0.0};
@@ -419,14 +474,18 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
       }
     }
     REnv cont;
-    rstep_children<TERMS, true>(P, inst, sd, 0, y0, y1, lane, spill, leafs, tv, acc, cont);
+    rstep_children_iz<TERMS>(P, inst, sd.nchild, cds, y0, y1, lane, spill, leafs, tv, acc, cont);
     e = cont;
+    sd = nsd;
+    nsd = nnsd;
+    mw = nmw;
+    cds[0] = ncds[0];
+    cds[1] = ncds[1];
   };
   for (int st = 0; st < ch.nsteps; st += 2) {
-    StepDev sd1;
-    run_step(st, sd0, sd1, fa, fb);
+    run_step(st, fa, fb);
     if (st + 1 >= ch.nsteps) break;
-    run_step(st + 1, sd1, sd0, fb, fa);
+    run_step(st + 1, fb, fa);
   }
   if (!TERMS) {
     acc = wave_sum(acc);
