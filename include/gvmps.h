@@ -186,9 +186,8 @@ int gvmps_batch_chain_flops(gvmps_batch *b, double *flops);
 /* Launch configuration of the forest-chain kernels: wavefronts per workgroup
  * (1, 2 or 4) and work-item order (0 instance-major, 1 (default) longest
  * chains first within each XCD's share of the instance-major items, so each
- * XCD keeps the same instances in its L2).  Order 2, the forest pass as one
- * persistent launch (k_rforest_persist), is rejected: its first GPU run did
- * not finish.  Results do not depend on the configuration.
+ * XCD keeps the same instances in its L2); other orders are rejected.
+ * Results do not depend on the configuration.
  * No reference counterpart
  * (the reference launches fixed shapes, sharded_mps.cuh:670-684); set by the
  * autotuner (gvmps/autotune.py). */
