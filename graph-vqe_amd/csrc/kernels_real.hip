@@ -53,24 +53,6 @@ __device__ __forceinline__ void renv_store(double *p, int lane, const REnv &e) {
   for (int t = 0; t < 4; ++t) p[a * 16 + kdx(g, t)] = e.v[t];
 }
 
-// Column-major slots (element (a, k) at k * 16 + a): each store or load
-// instruction covers four whole 128-byte rows, where the row-major form
-// touches sixteen.  Used for slots that only these kernels write and read
-// (the gradient's checkpointed step environments).  Each is written once by
-// the forward pass and read once by the backward pass, long after, so both
-// go non-temporal instead of displacing the hand-off slots in L2.
-__device__ __forceinline__ void renv_load_cm(const double *p, int lane, REnv &e) {
-  const int a = lane & 15, g = lane >> 4;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) e.v[t] = __builtin_nontemporal_load(p + kdx(g, t) * 16 + a);
-}
-
-__device__ __forceinline__ void renv_store_cm(double *p, int lane, const REnv &e) {
-  const int a = lane & 15, g = lane >> 4;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) __builtin_nontemporal_store(e.v[t], p + kdx(g, t) * 16 + a);
-}
-
 __device__ __forceinline__ void renv_unit(int lane, REnv &e) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) e.v[t] = (lane == 0 && t == 0) ? 1.0 : 0.0;
@@ -97,6 +79,68 @@ __device__ __forceinline__ void renv_load_sym(const double *p, int lane, REnv &e
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) e.v[t] = p[sym_idx(a, kdx(g, t))];
+}
+
+// The gradient's checkpointed step inputs are packed triangles too, written
+// once by the forward pass and read once by the backward pass much later, so
+// both go non-temporal.  The forward pass stores them contiguously: lane l
+// writes packed elements 2l and 2l+1, lanes 0-3 also 128 + 2l and 129 + 2l,
+// read from the wave's 16 x 17 LDS tile at the positions pack_tile_offsets
+// gives (in doubles); the backward pass gathers them with renv_load_sym_nt.
+struct PackTile {
+  int o[4];
+};
+typedef double gv_d2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ PackTile pack_tile_offsets(int lane) {
+  PackTile pt;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int p = q < 2 ? 2 * lane + q : min(128 + 2 * lane + (q - 2), 135);
+    int a = 0;  // row of packed element p: rows start at r (33 - r) / 2
+#pragma unroll
+    for (int r = 1; r < 16; ++r) a += p >= r * (33 - r) / 2 ? 1 : 0;
+    pt.o[q] = a * 17 + (p - a * (31 - a) / 2);
+  }
+  return pt;
+}
+
+// Symmetrised environment: each lane keeps its entries on and above the
+// diagonal and takes the mirror of those below it, through the wave's tile,
+// so E[a][k] and E[k][a] are the same bits.  The forward pass continues every
+// chain from the symmetrised input it checkpoints (STORE: into ck, as a
+// packed triangle, half the bytes of the square), and the backward pass
+// symmetrises the inputs it recomputes, so every checkpoint stride sees the
+// same environments.
+template <bool STORE>
+__device__ __forceinline__ void renv_symmetrize(REnv &e, int lane, double (*tl)[17], const PackTile *pt, double *ck) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) tl[a][kdx(g, t)] = e.v[t];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int k = kdx(g, t);
+    const double m = tl[k][a];
+    if (k < a) e.v[t] = m;
+  }
+  if (STORE && ck) {  // the upper triangle, packed, non-temporal
+    const double *tf = &tl[0][0];
+    gv_d2 *q = reinterpret_cast<gv_d2 *>(ck);
+    __builtin_nontemporal_store(gv_d2{tf[pt->o[0]], tf[pt->o[1]]}, q + lane);
+    if (lane < 4) __builtin_nontemporal_store(gv_d2{tf[pt->o[2]], tf[pt->o[3]]}, q + 64 + lane);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void renv_load_sym_nt(const double *p, int lane, REnv &e) {
+  const int a = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) e.v[t] = __builtin_nontemporal_load(p + sym_idx(a, kdx(g, t)));
 }
 
 // Real site slot layout: element (l, i, r) at (l*2 + i)*16 + r (512 doubles);
@@ -668,8 +712,10 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
                                            const double *__restrict__ rgarena, const double *__restrict__ rgarenaT,
                                            double2 *__restrict__ spills, double *__restrict__ parts,
                                            double2 *__restrict__ estore, double2 *__restrict__ gleafs,
-                                           double2 *__restrict__ leafs) {
+                                           double2 *__restrict__ leafs, double (*tr)[16][17]) {
   const int lane = threadIdx.x & 63;
+  double (*tl)[17] = tr[wave_id()];
+  const PackTile pt = pack_tile_offsets(lane);
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
@@ -691,10 +737,11 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
   RFrag fa, fb;
   rfrag_load(sites + (size_t)sd0.site * GV_SLOT, lane, fa);
   auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const RFrag &f, RFrag &nf) {
-    // checkpoint store first: the prefetch below is then the youngest memory
-    // operation when this step waits for f
+    // the step's input, symmetrised, is what it contracts and checkpoints
+    // (store first: the prefetch below is then the youngest memory operation
+    // when this step waits for f)
     const int cs = uload(P.ck_slot + ch.step0 + st);
-    if (cs >= 0) renv_store_cm(rslot(es, cs), lane, e);
+    renv_symmetrize<true>(e, lane, tl, &pt, cs >= 0 ? rslot(es, cs) : nullptr);
     nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));  // unconditional prefetch, as above
     rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
@@ -745,7 +792,8 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
                const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
                const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
                double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
-  rgfwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs);
+  __shared__ double tr[4][16][17];
+  rgfwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FWD_IZ_WAVES
@@ -753,7 +801,8 @@ k_rchains_gfwd_iz(const int2 *__restrict__ items, int nitems, const InstDev *__r
                   const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
                   const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
                   double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
-  rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs);
+  __shared__ double tr[4][16][17];
+  rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr);
 }
 
 // Backward (k_chains_bwd in real arithmetic).  For a step with input E and
@@ -798,12 +847,15 @@ __device__ __forceinline__ void rbwd_body(const int2 *__restrict__ items, int ni
     REnv E;
     const int cs = uload(P.ck_slot + step);
     if (cs >= 0) {
-      renv_load_cm(rslot(es, cs), lane, E);
+      renv_load_sym_nt(rslot(es, cs), lane, E);
     } else {  // recompute from the nearest checkpoint (step 0 of a chain is always kept)
       int s0 = st;
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
-      renv_load_cm(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
-      for (int q = s0; q < st; ++q) radvance(P, sites, ch.step0 + q, lane, E);
+      renv_load_sym_nt(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
+      for (int q = s0; q < st; ++q) {
+        radvance(P, sites, ch.step0 + q, lane, E);
+        renv_symmetrize<false>(E, lane, tr[wv], nullptr, nullptr);  // as the forward pass did before step q + 1
+      }
     }
     RFrag fa;  // M_i in the A-operand form
     rfrag_load(sites_a + (size_t)sd.site * GV_SLOT, lane, fa);
