@@ -260,13 +260,41 @@ __device__ __forceinline__ void rstep_children(const ProbDev &P, const InstDev &
   }
 }
 
+// A child environment's packed upper triangle through the wave's 16 x 17 LDS
+// tile: lane l gets packed elements 2l, 2l+1 (lo) and, lanes 0-3, 128 + 2l,
+// 129 + 2l (hi), at the tile positions ptab holds (4 per lane, from
+// pack_tile_offsets).  rpacked_store writes them with two contiguous stores.
+__device__ __forceinline__ void renv_pack_tile(const REnv &e, int lane, double (*tl)[17], const int *ptab, gv_d2 &lo,
+                                               gv_d2 &hi) {
+  const int a = lane & 15, g = lane >> 4;
+  const int4 o = *reinterpret_cast<const int4 *>(ptab + 4 * lane);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) tl[a][kdx(g, t)] = e.v[t];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double *tf = &tl[0][0];
+  lo = gv_d2{tf[o.x], tf[o.y]};
+  hi = gv_d2{tf[o.z], tf[o.w]};
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void rpacked_store(double *p, int lane, const gv_d2 &lo, const gv_d2 &hi) {
+  gv_d2 *q = reinterpret_cast<gv_d2 *>(p);
+  q[lane] = lo;
+  if (lane < 4) q[64 + lane] = hi;
+}
+
 // The children of an I/Z step (at most two: I and Z) from records the chain
 // loop fetched one step ahead, so no descriptor load waits here.
 template <bool TERMS>
 __device__ __forceinline__ void rstep_children_iz(const ProbDev &P, const InstDev &inst, int nchild,
                                                   const ChildDev (&cds)[2], const d4 &ya, const d4 &yb, int lane,
                                                   double2 *__restrict__ spill, double2 *__restrict__ leafs,
-                                                  double2 *__restrict__ tv, double &acc, REnv &cont) {
+                                                  double2 *__restrict__ tv, double &acc, REnv &cont,
+                                                  double (*tl)[17], const int *ptab) {
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     if (c >= nchild) break;
@@ -275,8 +303,15 @@ __device__ __forceinline__ void rstep_children_iz(const ProbDev &P, const InstDe
     REnv ce;
 #pragma unroll
     for (int v = 0; v < 4; ++v) ce.v[v] = __builtin_fma(sgn, yb[v], ya[v]);
-    if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
-    if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
+    if (cd.store >= 0 || cd.defer >= 0 || cd.action >= 0) {
+      // the packed triangle through the wave's tile, so each target takes
+      // two contiguous stores instead of four masked gathers of the square
+      gv_d2 lo, hi;
+      renv_pack_tile(ce, lane, tl, ptab, lo, hi);
+      if (cd.store >= 0) rpacked_store(rslot(spill, cd.store), lane, lo, hi);
+      if (cd.defer >= 0) rpacked_store(rslot(leafs, inst.leaf_off + cd.defer), lane, lo, hi);
+      if (cd.action >= 0) rpacked_store(rslot(spill, cd.action), lane, lo, hi);
+    }
     const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
 #pragma unroll(GV_IZ_TERM_UNROLL)
     for (int t = 0; t < nterm_inline; ++t) {
@@ -295,7 +330,6 @@ __device__ __forceinline__ void rstep_children_iz(const ProbDev &P, const InstDe
       }
     }
     if (cd.action == -2) cont = ce;
-    else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
   }
 }
 
@@ -425,7 +459,7 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
                                                    const double *__restrict__ farenaT,
                                                    const unsigned *__restrict__ smask, double2 *__restrict__ leafs,
                                                    double2 *__restrict__ spills, double2 *__restrict__ terms,
-                                                   double *__restrict__ parts) {
+                                                   double *__restrict__ parts, double (*tl)[17], const int *ptab) {
   const InstDev inst = uload(insts + it.x);
   const ProbDev P = uload(probs + inst.prob);
   const ChainDev ch = uload(P.chains + it.y);
@@ -518,7 +552,7 @@ __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane
       }
     }
     REnv cont;
-    rstep_children_iz<TERMS>(P, inst, sd.nchild, cds, y0, y1, lane, spill, leafs, tv, acc, cont);
+    rstep_children_iz<TERMS>(P, inst, sd.nchild, cds, y0, y1, lane, spill, leafs, tv, acc, cont, tl, ptab);
     e = cont;
     sd = nsd;
     nsd = nnsd;
@@ -555,13 +589,22 @@ __device__ __forceinline__ void rchains_merged_body(const int2 *__restrict__ ite
                                                     const int *__restrict__ cflags, double2 *__restrict__ leafs,
                                                     double2 *__restrict__ spills, double2 *__restrict__ terms,
                                                     double *__restrict__ parts) {
+  // per-wave tiles for the packed child stores and the lanes' tile positions
+  __shared__ double tiles[4][16][17];
+  __shared__ int ptab[64 * 4];
+  if (threadIdx.x < 64) {
+    const PackTile pt = pack_tile_offsets(threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ptab[4 * threadIdx.x + q] = pt.o[q];
+  }
+  __syncthreads();
   const int w = xcd_item();
   if (w >= nitems) return;
   const int2 it = uload(items + w);
   const InstDev inst = uload(insts + it.x);
   if (uload(cflags + inst.part_off + it.y))
     rchain_item_merged<TERMS>(it, threadIdx.x & 63, insts, probs, rarena, rarenaT, farena, farenaT, smask, leafs,
-                              spills, terms, parts);
+                              spills, terms, parts, tiles[wave_id()], ptab);
   else
     rchain_item<TERMS, true>(it, threadIdx.x & 63, insts, probs, rarena, rarenaT, nullptr, leafs, spills, terms,
                              parts);
