@@ -113,7 +113,7 @@ __device__ __forceinline__ PackTile pack_tile_offsets(int lane) {
 // symmetrises the inputs it recomputes, so every checkpoint stride sees the
 // same environments.
 template <bool STORE>
-__device__ __forceinline__ void renv_symmetrize(REnv &e, int lane, double (*tl)[17], const PackTile *pt, double *ck) {
+__device__ __forceinline__ void renv_symmetrize(REnv &e, int lane, double (*tl)[17], const int *ptab, double *ck) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) tl[a][kdx(g, t)] = e.v[t];
@@ -127,10 +127,11 @@ __device__ __forceinline__ void renv_symmetrize(REnv &e, int lane, double (*tl)[
     if (k < a) e.v[t] = m;
   }
   if (STORE && ck) {  // the upper triangle, packed, non-temporal
+    const int4 o = *reinterpret_cast<const int4 *>(ptab + 4 * lane);
     const double *tf = &tl[0][0];
     gv_d2 *q = reinterpret_cast<gv_d2 *>(ck);
-    __builtin_nontemporal_store(gv_d2{tf[pt->o[0]], tf[pt->o[1]]}, q + lane);
-    if (lane < 4) __builtin_nontemporal_store(gv_d2{tf[pt->o[2]], tf[pt->o[3]]}, q + 64 + lane);
+    __builtin_nontemporal_store(gv_d2{tf[o.x], tf[o.y]}, q + lane);
+    if (lane < 4) __builtin_nontemporal_store(gv_d2{tf[o.z], tf[o.w]}, q + 64 + lane);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -279,6 +280,17 @@ __device__ __forceinline__ void renv_pack_tile(const REnv &e, int lane, double (
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The table of tile positions, filled by the workgroup's first wave before
+// any wave returns (a workgroup barrier).
+__device__ __forceinline__ void pack_table(int *ptab) {
+  if (threadIdx.x < 64) {
+    const PackTile pt = pack_tile_offsets(threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ptab[4 * threadIdx.x + q] = pt.o[q];
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void rpacked_store(double *p, int lane, const gv_d2 &lo, const gv_d2 &hi) {
@@ -592,12 +604,7 @@ __device__ __forceinline__ void rchains_merged_body(const int2 *__restrict__ ite
   // per-wave tiles for the packed child stores and the lanes' tile positions
   __shared__ double tiles[4][16][17];
   __shared__ int ptab[64 * 4];
-  if (threadIdx.x < 64) {
-    const PackTile pt = pack_tile_offsets(threadIdx.x);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) ptab[4 * threadIdx.x + q] = pt.o[q];
-  }
-  __syncthreads();
+  pack_table(ptab);
   const int w = xcd_item();
   if (w >= nitems) return;
   const int2 it = uload(items + w);
@@ -755,10 +762,9 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
                                            const double *__restrict__ rgarena, const double *__restrict__ rgarenaT,
                                            double2 *__restrict__ spills, double *__restrict__ parts,
                                            double2 *__restrict__ estore, double2 *__restrict__ gleafs,
-                                           double2 *__restrict__ leafs, double (*tr)[16][17]) {
+                                           double2 *__restrict__ leafs, double (*tr)[16][17], const int *ptab) {
   const int lane = threadIdx.x & 63;
   double (*tl)[17] = tr[wave_id()];
-  const PackTile pt = pack_tile_offsets(lane);
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7;
   const int logical = xcd * q8 + min(xcd, r8) + (b >> 3);
@@ -784,7 +790,7 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
     // (store first: the prefetch below is then the youngest memory operation
     // when this step waits for f)
     const int cs = uload(P.ck_slot + ch.step0 + st);
-    renv_symmetrize<true>(e, lane, tl, &pt, cs >= 0 ? rslot(es, cs) : nullptr);
+    renv_symmetrize<true>(e, lane, tl, ptab, cs >= 0 ? rslot(es, cs) : nullptr);
     nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));  // unconditional prefetch, as above
     rfrag_load(sites + (size_t)nsd.site * GV_SLOT, lane, nf);
     const d4 x0 = rprod_x(e, f, 0), x1 = rprod_x(e, f, 1);
@@ -800,9 +806,16 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
         REnv ce;
 #pragma unroll
         for (int v = 0; v < 4; ++v) ce.v[v] = __builtin_fma(cd.kind == CK_Z ? -1.0 : 1.0, yb[v], ya[v]);  // ya +- yb
-        if (cd.store >= 0) renv_store_sym(rslot(spill, cd.store), lane, ce);
-        if (cd.gleaf >= 0) renv_store_sym(rslot(gl, cd.gleaf), lane, ce);
-        if (cd.defer >= 0) renv_store_sym(rslot(leafs, inst.leaf_off + cd.defer), lane, ce);
+        if (cd.store >= 0 || cd.gleaf >= 0 || cd.defer >= 0 || cd.action >= 0) {
+          // the packed triangle through the wave's tile: two contiguous
+          // stores per target (as in rstep_children_iz)
+          gv_d2 lo, hi;
+          renv_pack_tile(ce, lane, tl, ptab, lo, hi);
+          if (cd.store >= 0) rpacked_store(rslot(spill, cd.store), lane, lo, hi);
+          if (cd.gleaf >= 0) rpacked_store(rslot(gl, cd.gleaf), lane, lo, hi);
+          if (cd.defer >= 0) rpacked_store(rslot(leafs, inst.leaf_off + cd.defer), lane, lo, hi);
+          if (cd.action >= 0) rpacked_store(rslot(spill, cd.action), lane, lo, hi);
+        }
         const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
         for (int t = 0; t < nterm_inline; ++t) {
           const TermEnd te = uload(P.term_list + cd.term0 + t);
@@ -815,7 +828,6 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
           acc += te.cre * sr;  // Re(c_k) <P_k>
         }
         if (cd.action == -2) cont = ce;
-        else if (cd.action >= 0) renv_store_sym(rslot(spill, cd.action), lane, ce);
       }
     }
     e = cont;
@@ -836,7 +848,9 @@ k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__rest
                const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
                double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
   __shared__ double tr[4][16][17];
-  rgfwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr);
+  __shared__ int ptab[64 * 4];
+  pack_table(ptab);
+  rgfwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr, ptab);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_FWD_IZ_WAVES
@@ -845,7 +859,9 @@ k_rchains_gfwd_iz(const int2 *__restrict__ items, int nitems, const InstDev *__r
                   const double *__restrict__ rgarenaT, double2 *__restrict__ spills, double *__restrict__ parts,
                   double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
   __shared__ double tr[4][16][17];
-  rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr);
+  __shared__ int ptab[64 * 4];
+  pack_table(ptab);
+  rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr, ptab);
 }
 
 // Backward (k_chains_bwd in real arithmetic).  For a step with input E and
