@@ -100,6 +100,10 @@ struct ChildDev {
 // Shallow prefix-forest depths with at most this many chains per instance run
 // in the suffix forest's energy launches (their term ends deferred).
 #define GV_MERGE_CHAINS 64
+// Steps of a root chain before its heavy path continues in the next generation.
+#ifndef GV_ROOT_STEPS
+#define GV_ROOT_STEPS 5
+#endif
 struct LeafDev {
   int term0, nterm;
 };
