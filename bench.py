@@ -210,8 +210,8 @@ def main():
     plain_rate, same = plain_rate_of(lambda b, v: b.energies(v), Batch([prob], [1]), x[: args.plain_sample],
                                      lambda i, r: torch.equal(r, e[i:i + 1]))
 
-    # dominant kernel: the forest-chain kernel (its 6 generation launches per
-    # step), timed per launch by HIP events around each launch on the launch
+    # dominant kernel: the forest-chain kernel (one launch per generation,
+    # 8 per 40-qubit step), timed per launch by HIP events around each launch on the launch
     # stream (slots 6, 7); the chains phase adds the site packing and the
     # leaf term-end launch
     chains_ms = phase_ms[2] / max(calls, 1)

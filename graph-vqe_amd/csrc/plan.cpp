@@ -406,7 +406,7 @@ void forest_chains(Forest &F, int n, int gen0, int &nspill, double &flops, std::
       // a root chain (the only work of the first launch, a few waves per
       // instance) hands its heavy path on to the next generation after
       // GV_ROOT_STEPS steps, so the first launch is short and its
-      // continuation runs beside the other chains of the second
+      // continuation runs beside the other chains of the second generation
       if (st.node < 0 && (int)co.steps.size() + 1 >= GV_ROOT_STEPS) cont = -1;
       for (int kd = 0; kd < 4; ++kd) {
         const int c = kids[kd];
