@@ -269,6 +269,7 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
       part += (long long)P.chains.size() + (long long)P.leaves.size();
       B->max_gen = std::max(B->max_gen, P.ngen);
       B->max_ggen = std::max(B->max_ggen, P.nggen);
+      B->max_bgen = std::max(B->max_bgen, P.nbgen);
     }
   }
   if (B->inst.empty()) throw std::runtime_error("empty batch");
@@ -351,6 +352,20 @@ Batch *make_batch(gvmps_problem *const *problems, const int *counts, int n_probl
     B->gitems_host.push_back(items);
     xcd_longest_first(items, B->wpb, chain_len(*B));
     B->gitems_len.push_back(upload(B->owned, items.data(), items.size()));
+  }
+  for (int g = 0; g < B->max_bgen; ++g) {
+    std::vector<int2> items;
+    for (size_t i = 0; i < B->inst.size(); ++i) {
+      const Problem &P = *B->probs[B->inst[i].prob];
+      if (g >= P.nbgen) continue;
+      for (int q = P.bgen_begin[g]; q < P.bgen_begin[g + 1]; ++q)
+        items.push_back(make_int2((int)i, P.bgen_chains[q]));
+    }
+    B->bitems.push_back(upload(B->owned, items.data(), items.size()));
+    B->bn_items.push_back((int)items.size());
+    B->bitems_host.push_back(items);
+    xcd_longest_first(items, B->wpb, chain_len(*B));
+    B->bitems_len.push_back(upload(B->owned, items.data(), items.size()));
   }
   B->set_checkpoint(1);
   return B.release();
@@ -619,11 +634,11 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
     B.mark(2, stream);
     hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 1), "k_reduce launch");
     B.mark(3, stream);
-    for (int g = B.max_ggen - 1; g >= 0; --g) {
-      const int ni = B.gn_items[g];
+    for (int g = B.max_bgen - 1; g >= 0; --g) {
+      const int ni = B.bn_items[g];
       if (ni == 0) continue;
       B.kbegin(stream);
-      hip_check(kl_rchains_bwd(ni, B.wpb, B.iz, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
+      hip_check(kl_rchains_bwd(ni, B.wpb, B.iz, stream, B.bgen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
                                B.spills, B.adjspill,
                                B.estore, B.gleafs, B.aslot),
                 "k_rchains_bwd launch");
@@ -652,11 +667,11 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   B.mark(2, stream);
   hip_check(kl_reduce_energy(ninst, stream, B.d_inst, B.d_probs, B.parts, B.energies, 0), "k_reduce launch");
   B.mark(3, stream);
-  for (int g = B.max_ggen - 1; g >= 0; --g) {
-    const int ni = B.gn_items[g];
+  for (int g = B.max_bgen - 1; g >= 0; --g) {
+    const int ni = B.bn_items[g];
     if (ni == 0) continue;
     B.kbegin(stream);
-    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.ggen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
+    hip_check(kl_chains_bwd(ni, B.wpb, stream, B.bgen_items(g), B.d_inst, B.d_probs, B.garena, B.n_site_total, B.spills,
                             B.adjspill, B.estore,
                             B.gleafs, B.aslot),
               "k_chains_bwd launch");
@@ -952,6 +967,12 @@ int gvmps_batch_set_launch(gvmps_batch *b, int waves_per_group, int order) {
         std::vector<int2> items = B.gitems_host[g];
         xcd_longest_first(items, waves_per_group, chain_len(B));
         hip_check(hipMemcpy(B.gitems_len[g], items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+      }
+      for (size_t g = 0; g < B.bitems_host.size(); ++g) {
+        std::vector<int2> items = B.bitems_host[g];
+        xcd_longest_first(items, waves_per_group, chain_len(B));
+        hip_check(hipMemcpy(B.bitems_len[g], items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice),
                   "hipMemcpy");
       }
     }

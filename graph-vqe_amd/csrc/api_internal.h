@@ -44,12 +44,18 @@ struct Batch {
   std::vector<std::vector<int2>> items_host;  // instance-major, host copy
   std::vector<int> n_items;
   int max_gen = 0;
-  // reverse-gradient generations (suffix forest, then prefix forest)
+  // complex gradient forward generations (suffix forest, then prefix forest)
   std::vector<int2 *> gitems;
   std::vector<int2 *> gitems_len;
   std::vector<std::vector<int2>> gitems_host;
   std::vector<int> gn_items;
   int max_ggen = 0;
+  // backward launches (the two forests' backward passes side by side)
+  std::vector<int2 *> bitems;
+  std::vector<int2 *> bitems_len;
+  std::vector<std::vector<int2>> bitems_host;
+  std::vector<int> bn_items;
+  int max_bgen = 0;
   // launch configuration of the chain kernels (set by the autotuner)
   int wpb = 4;    // wavefronts per workgroup: 1, 2 or 4
   int order = 1;  // 0: instance-major items, 1: longest chains first within each XCD's share
@@ -66,6 +72,7 @@ struct Batch {
   bool use_smask = true;      // chain steps skip exact-zero products (gvmps_batch_set_sparsity)
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   const int2 *ggen_items(int g) const { return order ? gitems_len[g] : gitems[g]; }
+  const int2 *bgen_items(int g) const { return order ? bitems_len[g] : bitems[g]; }
   // captured evaluate + sample sequence
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;

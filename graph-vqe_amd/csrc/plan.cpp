@@ -477,8 +477,9 @@ void build_chains(Problem &P) {
   for (const ChainOut &c : chains) right_gens = std::max(right_gens, c.gen + 1);
   const size_t nright = chains.size();
   forest_chains(L, n, right_gens, P.nspill, P.flops_exec, chains);
-  // Reverse-gradient generations: the suffix forest, then the prefix forest
-  // (its inline term ends read suffix-forest slots).  Energy launches: the
+  // Generations of the complex gradient's forward pass: the suffix forest,
+  // then the prefix forest (its inline term ends read suffix-forest slots).
+  // Energy launches (and the real gradient's forward pass): the
   // shallow prefix depths with few chains (latency-bound launches) run in the
   // suffix forest's launches of the same depth, with every term end deferred
   // to the leaf pass, which runs after both forests.
@@ -562,6 +563,26 @@ void build_chains(Problem &P) {
     P.ggen_begin.push_back((int)P.ggen_chains.size());
   }
   P.nggen = nggen;
+  // backward launches: a chain's backward needs the adjoints its spilled
+  // children hand back (later generations of its own forest) and forward
+  // values only (checkpoints, suffix slots, the prefix leaves of the term
+  // ends that read its slots), so the two forests' backward passes are
+  // independent.  They start together at their deepest generations: the
+  // forest with fewer generations then runs its latency-bound root depths
+  // (few chains, long heavy paths) beside the other forest's wide ones.
+  const int dp = nggen - right_gens, kb = std::max(right_gens, dp);
+  std::vector<std::vector<int>> byb(kb);
+  for (int idx : order) {
+    const int g = ggen[idx];
+    byb[(size_t)idx < nright ? g + kb - right_gens : g - right_gens + kb - dp].push_back(pos[idx]);
+  }
+  P.bgen_begin.assign(1, 0);
+  P.bgen_chains.clear();
+  for (const auto &v : byb) {
+    P.bgen_chains.insert(P.bgen_chains.end(), v.begin(), v.end());
+    P.bgen_begin.push_back((int)P.bgen_chains.size());
+  }
+  P.nbgen = kb;
 }
 
 // Per-site programs of the bond-doubling state used by the reverse gradient

@@ -46,6 +46,8 @@ struct Problem {
   std::vector<int> gen_begin;            // energy launch g: chains [gen_begin[g], gen_begin[g+1])
   std::vector<int> ggen_begin, ggen_chains;  // gradient generation g: chains ggen_chains[ggen_begin[g] ..)
   int nggen = 0;
+  std::vector<int> bgen_begin, bgen_chains;  // backward launch g (run from the last to 0): chains bgen_chains[bgen_begin[g] ..)
+  int nbgen = 0;
   int nspill = 0, ngen = 0, n_nodes = 0;
   std::vector<LeafDev> leaves;           // deferred term-end leaves (energy passes)
   // reverse gradient
