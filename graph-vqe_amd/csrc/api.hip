@@ -450,7 +450,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
     hip_check(kl_pack_real(B.n_site_total, stream, B.arena, B.rarena), "k_pack_real launch");
     if (B.use_smask) {
       hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask, B.rfarena), "k_site_masks launch");
-      hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags),
+      hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags, 0),
                 "k_chain_flags launch");
     }
     for (int g = 0; g < B.max_gen; ++g) {
@@ -619,13 +619,21 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
   if (B.real && B.arith == 0) {
     // real tape, no Y words: the forests and their adjoints in real arithmetic
     hip_check(kl_pack_real(B.n_site_total, stream, B.garena, B.rarena), "k_pack_real launch");
+    // the zero structure of the bond-doubling state for the I/Z forward chains
+    const bool gs = B.iz && B.use_smask;
+    if (gs) {
+      hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask, B.rfarena), "k_site_masks launch");
+      hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags, 1),
+                "k_chain_flags launch");
+    }
     B.mark(1, stream);
     // the energy launches, deferred term ends by the leaf kernel
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
       if (ni == 0) continue;
       hip_check(kl_rchains_gfwd(ni, B.wpb, B.iz, stream, B.gen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
-                                B.spills, B.parts, B.estore, B.gleafs, B.leafs),
+                                B.spills, B.parts, B.estore, B.gleafs, B.leafs, B.rfarena, gs ? B.smask : nullptr,
+                                B.cflags),
                 "k_rchains_gfwd launch");
     }
     hip_check(kl_rleaf_terms(B.n_leaf_items, stream, B.leaf_desc, B.leafs, B.spills, nullptr,
@@ -639,8 +647,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
       if (ni == 0) continue;
       B.kbegin(stream);
       hip_check(kl_rchains_bwd(ni, B.wpb, B.iz, stream, B.bgen_items(g), B.d_inst, B.d_probs, B.rarena, B.n_site_total,
-                               B.spills, B.adjspill,
-                               B.estore, B.gleafs, B.aslot),
+                               B.spills, B.adjspill, B.estore, B.gleafs, B.aslot, gs ? B.smask : nullptr, B.cflags),
                 "k_rchains_bwd launch");
       B.kend(stream);
     }

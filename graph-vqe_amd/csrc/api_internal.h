@@ -146,15 +146,17 @@ hipError_t kl_rchains(int nitems, int wpb, bool iz, hipStream_t s, const int2 *i
                       double *parts);
 hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, unsigned *smask, double *farena);
 hipError_t kl_chain_flags(int ninst, int max_chains, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                          const unsigned *smask, int *cflags);
+                          const unsigned *smask, int *cflags, int grad);
 hipError_t kl_rleaf_terms(int nitems, hipStream_t s, const LeafItem *items, const double2 *leafs,
                           const double2 *spills, double2 *terms, double *parts);
 hipError_t kl_rchains_gfwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
-                           double2 *estore, double2 *gleafs, double2 *leafs);
+                           double2 *estore, double2 *gleafs, double2 *leafs, const double *farena,
+                           const unsigned *smask, const int *cflags);
 hipError_t kl_rchains_bwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                           const ProbDev *probs, const double *rgarena, long long nsite, const double2 *spills,
-                          double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot);
+                          double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot,
+                          const unsigned *smask, const int *cflags);
 hipError_t kl_rsite_reduce(int ninst, int max_n, hipStream_t s, const InstDev *insts, const ProbDev *probs,
                            const double2 *aslot, double2 *abar);
 // kernel launchers (kernels_grad.hip)

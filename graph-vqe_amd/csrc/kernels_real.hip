@@ -112,19 +112,35 @@ __device__ __forceinline__ PackTile pack_tile_offsets(int lane) {
 // packed triangle, half the bytes of the square), and the backward pass
 // symmetrises the inputs it recomputes, so every checkpoint stride sees the
 // same environments.
+// Row order of the parity-split products: 0, 2, .., 14, 1, 3, .., 15 (row x of
+// the permuted matrices is row par_perm(x) of the natural ones).
+__device__ __forceinline__ constexpr int par_perm(int x) { return x < 8 ? 2 * x : 2 * x - 15; }
+
+// perm: e leaves as P E P^T (both indices in par_perm order) instead, the
+// form the parity-split products contract; the checkpoint stays natural.
 template <bool STORE>
-__device__ __forceinline__ void renv_symmetrize(REnv &e, int lane, double (*tl)[17], const int *ptab, double *ck) {
+__device__ __forceinline__ void renv_symmetrize(REnv &e, int lane, double (*tl)[17], const int *ptab, double *ck,
+                                                bool perm = false) {
   const int a = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 4; ++t) tl[a][kdx(g, t)] = e.v[t];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (perm) {
+    const int pa = par_perm(a);
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int k = kdx(g, t);
-    const double m = tl[k][a];
-    if (k < a) e.v[t] = m;
+    for (int t = 0; t < 4; ++t) {
+      const int pk = par_perm(kdx(g, t));
+      e.v[t] = tl[min(pa, pk)][max(pa, pk)];
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = kdx(g, t);
+      const double m = tl[k][a];
+      if (k < a) e.v[t] = m;
+    }
   }
   if (STORE && ck) {  // the upper triangle, packed, non-temporal
     const int4 o = *reinterpret_cast<const int4 *>(ptab + 4 * lane);
@@ -152,6 +168,15 @@ __device__ __forceinline__ void rfrag_load(const double *slot, int lane, RFrag &
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int t = 0; t < 4; ++t) f.v[i][t] = slot[(kdx(g, t) * 2 + i) * 16 + j];
+}
+
+// rfrag_load with the rows in par_perm order.
+__device__ __forceinline__ void rfrag_load_perm(const double *slot, int lane, RFrag &f) {
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) f.v[i][t] = slot[(par_perm(kdx(g, t)) * 2 + i) * 16 + j];
 }
 
 // X_i = E M_i (D layout), the Gauss form's t1 of prod_x.
@@ -463,6 +488,12 @@ __device__ __forceinline__ void ffrag_load(const double *slot, int lane, double 
   for (int t = 0; t < 4; ++t) f[t] = slot[kdx(g, t) * 16 + j];
 }
 
+__device__ __forceinline__ void ffrag_load_perm(const double *slot, int lane, double (&f)[4]) {
+  const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) f[t] = slot[par_perm(kdx(g, t)) * 16 + j];
+}
+
 template <bool TERMS>
 __device__ __forceinline__ void rchain_item_merged(const int2 it, const int lane, const InstDev *__restrict__ insts,
                                                    const ProbDev *__restrict__ probs,
@@ -696,11 +727,21 @@ __device__ __forceinline__ int sidx(int l, int i, int r) { return (l * 2 + i) * 
 
 // Input environment of the step after `step` in its chain (the continuing
 // child), with the products and combination of k_rchains_gfwd: bit for bit
-// the environment the forward pass had.
-__device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, int step, int lane, REnv &e) {
+// the environment the forward pass had.  smc: the mask words of the chain's
+// copy when the forward ran it over the zero structure (a suffix chain's
+// parity-split steps contracted P E P^T with the rows of M_i in par_perm
+// order: here the full products in that order, which add the same nonzero
+// terms in the same order), else null; tl: the wave's LDS tile.
+__device__ __forceinline__ void radvance(const ProbDev &P, const double *sites, int step, int lane, REnv &e,
+                                         const unsigned *smc, double (*tl)[17]) {
   const StepDev sd = uload(P.steps + step);
   RFrag f;
-  rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, f);
+  if (smc && (uload(smc + 2 * sd.site) & GV_SM_PARITY)) {
+    rfrag_load_perm(sites + (size_t)sd.site * GV_SLOT, lane, f);
+    renv_symmetrize<false>(e, lane, tl, nullptr, nullptr, true);  // e is symmetric: P E P^T
+  } else {
+    rfrag_load(sites + (size_t)sd.site * GV_SLOT, lane, f);
+  }
   int kind = CK_I;
   for (int c = 0; c < sd.nchild; ++c) {
     const ChildDev cd = uload(P.children + sd.child0 + c);
@@ -842,6 +883,135 @@ __device__ __forceinline__ void rgfwd_body(const int2 *__restrict__ items, int n
   if (lane == 0) parts[inst.part_off + it.y] = acc;
 }
 
+// The forward of an I/Z chain over the zero structure of the bond-doubling
+// state (its chain flag set by k_chain_flags in gradient mode): a prefix
+// chain's steps on sites with disjoint columns take the merged products over
+// F = M_0 + M_1, a suffix chain's steps on parity-split sites the row-split
+// products over P F and P E P^T (the rows in par_perm order: M_0 in rows 0-7,
+// M_1 in rows 8-15), each in half the MFMAs; other steps take the full
+// products of rgfwd_body.  The merged products give the bits of the full
+// ones; the permuted ones those of the full products in the permuted order,
+// which radvance repeats.  Checkpoints, stores and term ends as rgfwd_body.
+__device__ __forceinline__ void rgfwd_struct_item(const int2 it, const int lane, const InstDev *__restrict__ insts,
+                                                  const ProbDev *__restrict__ probs,
+                                                  const double *__restrict__ rgarena,
+                                                  const double *__restrict__ rgarenaT,
+                                                  const double *__restrict__ farena,
+                                                  const double *__restrict__ farenaT,
+                                                  const unsigned *__restrict__ smask, double2 *__restrict__ spills,
+                                                  double *__restrict__ parts, double2 *__restrict__ estore,
+                                                  double2 *__restrict__ gleafs, double2 *__restrict__ leafs,
+                                                  double (*tl)[17], const int *ptab) {
+  const InstDev inst = uload(insts + it.x);
+  const ProbDev P = uload(probs + inst.prob);
+  const ChainDev ch = uload(P.chains + it.y);
+  const bool dir1 = ch.dir != 0;
+  const long long slot0 = inst.site_off / GV_SLOT;
+  const double *sites = (dir1 ? rgarenaT : rgarena) + inst.site_off;
+  const double *fsites = (dir1 ? farenaT : farena) + slot0 * GV_FSLOT;
+  const unsigned *sm = smask + slot0 * 2 + (dir1 ? 1 : 0);
+  const int a16 = lane & 15, g4 = lane >> 4;
+  double2 *spill = spills + inst.spill_off;
+  double2 *es = estore + inst.ck_off * GV_ENV;
+  double2 *gl = gleafs + inst.gleaf_off * GV_ENV;
+  REnv e;
+  if (ch.src >= 0) renv_load_sym(rslot(spill, ch.src), lane, e);
+  else renv_unit(lane, e);
+  double acc = 0.0;
+  // a suffix chain's fragments are loaded in par_perm row order
+  auto fload = [&](int site, double(&f)[4]) {
+    if (dir1) ffrag_load_perm(fsites + (size_t)site * GV_FSLOT, lane, f);
+    else ffrag_load(fsites + (size_t)site * GV_FSLOT, lane, f);
+  };
+  StepDev sd0 = uload(P.steps + ch.step0);
+  double fa[4], fb[4];
+  fload(sd0.site, fa);
+  auto run_step = [&](int st, const StepDev &sd, StepDev &nsd, const double(&f)[4], double(&nf)[4]) {
+    const unsigned mw = uload(sm + 2 * sd.site);
+    const bool merged = !dir1 && (mw & GV_SM_DISJOINT) != 0u;
+    const bool perm = dir1 && (mw & GV_SM_PARITY) != 0u;
+    const int cs = uload(P.ck_slot + ch.step0 + st);
+    renv_symmetrize<true>(e, lane, tl, ptab, cs >= 0 ? rslot(es, cs) : nullptr, perm);
+    nsd = uload(P.steps + ch.step0 + min(st + 1, ch.nsteps - 1));  // unconditional prefetch
+    fload(nsd.site, nf);
+    d4 y0, y1;
+    if (merged) {
+      d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) x = mfma(e.v[t], f[t], x);
+      d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) y = mfma(x[t], f[t], y);
+      rsplit(y, mw, 0, a16, g4, y0, y1);
+    } else if (perm) {  // P M_0 in rows 0-7 (f[0], f[1]), P M_1 in rows 8-15 (f[2], f[3])
+      d4 p0 = {0.0, 0.0, 0.0, 0.0}, p1 = {0.0, 0.0, 0.0, 0.0};
+      p0 = mfma(e.v[0], f[0], p0);
+      p1 = mfma(e.v[2], f[2], p1);
+      p0 = mfma(e.v[1], f[1], p0);
+      p1 = mfma(e.v[3], f[3], p1);
+      d4 q0 = {0.0, 0.0, 0.0, 0.0}, q1 = {0.0, 0.0, 0.0, 0.0};
+      q0 = mfma(p0[0], f[0], q0);
+      q1 = mfma(p1[2], f[2], q1);
+      q0 = mfma(p0[1], f[1], q0);
+      q1 = mfma(p1[3], f[3], q1);
+      y0 = q0;
+      y1 = q1;
+    } else {  // no structure: the components from the full slot, one at a time
+      const double *slot = sites + (size_t)sd.site * GV_SLOT;
+      double m[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) m[t] = slot[(kdx(g4, t) * 2 + i) * 16 + a16];
+        d4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x = mfma(e.v[t], m[t], x);
+        d4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) y = mfma(x[t], m[t], y);
+        if (i == 0) y0 = y;
+        else y1 = y;
+      }
+    }
+    REnv cont;
+    for (int c = 0; c < sd.nchild; ++c) {
+      const ChildDev cd = uload(P.children + sd.child0 + c);
+      REnv ce;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ce.v[v] = __builtin_fma(cd.kind == CK_Z ? -1.0 : 1.0, y1[v], y0[v]);  // y0 +- y1
+      if (cd.store >= 0 || cd.gleaf >= 0 || cd.defer >= 0 || cd.action >= 0) {
+        gv_d2 lo, hi;
+        renv_pack_tile(ce, lane, tl, ptab, lo, hi);
+        if (cd.store >= 0) rpacked_store(rslot(spill, cd.store), lane, lo, hi);
+        if (cd.gleaf >= 0) rpacked_store(rslot(gl, cd.gleaf), lane, lo, hi);
+        if (cd.defer >= 0) rpacked_store(rslot(leafs, inst.leaf_off + cd.defer), lane, lo, hi);
+        if (cd.action >= 0) rpacked_store(rslot(spill, cd.action), lane, lo, hi);
+      }
+      const int nterm_inline = cd.defer >= 0 ? 0 : cd.nterm;
+      for (int t = 0; t < nterm_inline; ++t) {
+        const TermEnd te = uload(P.term_list + cd.term0 + t);
+        REnv r;
+        if (te.rslot >= 0) renv_load_sym(rslot(spill, te.rslot), lane, r);
+        else renv_unit(lane, r);
+        double sr = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sr += ce.v[v] * r.v[v];
+        acc += te.cre * sr;  // Re(c_k) <P_k>
+      }
+      if (cd.action == -2) cont = ce;
+    }
+    e = cont;
+  };
+  for (int st = 0; st < ch.nsteps; st += 2) {
+    StepDev sd1;
+    run_step(st, sd0, sd1, fa, fb);
+    if (st + 1 >= ch.nsteps) break;
+    run_step(st + 1, sd1, sd0, fb, fa);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) parts[inst.part_off + it.y] = acc;
+}
+
 extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_gfwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
                const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
@@ -864,6 +1034,28 @@ k_rchains_gfwd_iz(const int2 *__restrict__ items, int nitems, const InstDev *__r
   rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr, ptab);
 }
 
+// I/Z forests with the zero structure: flagged chains run rgfwd_struct_item.
+extern "C" __global__ void __launch_bounds__(256) GV_FWD_IZ_WAVES
+k_rchains_gfwd_izs(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
+                   const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
+                   const double *__restrict__ rgarenaT, const double *__restrict__ farena,
+                   const double *__restrict__ farenaT, const unsigned *__restrict__ smask,
+                   const int *__restrict__ cflags, double2 *__restrict__ spills, double *__restrict__ parts,
+                   double2 *__restrict__ estore, double2 *__restrict__ gleafs, double2 *__restrict__ leafs) {
+  __shared__ double tr[4][16][17];
+  __shared__ int ptab[64 * 4];
+  pack_table(ptab);
+  const int w = xcd_item();
+  if (w >= nitems) return;
+  const int2 it = uload(items + w);
+  const InstDev inst = uload(insts + it.x);
+  if (uload(cflags + inst.part_off + it.y))
+    rgfwd_struct_item(it, threadIdx.x & 63, insts, probs, rgarena, rgarenaT, farena, farenaT, smask, spills, parts,
+                      estore, gleafs, leafs, tr[wave_id()], ptab);
+  else
+    rgfwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, parts, estore, gleafs, leafs, tr, ptab);
+}
+
 // Backward (k_chains_bwd in real arithmetic).  For a step with input E and
 // site matrices M_i: Ybar00 = I+Z, Ybar11 = I-Z, Ybar01 = Ybar10 = X (child
 // adjoints), G_i = sum_j M_j Ybar_ji, Ebar = sum_i G_i M_i^T, Mbar_i = 2 E G_i.
@@ -874,7 +1066,8 @@ __device__ __forceinline__ void rbwd_body(const int2 *__restrict__ items, int ni
                                           const double *__restrict__ rgarena, const double *__restrict__ rgarenaT,
                                           const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
                                           const double2 *__restrict__ estore, const double2 *__restrict__ gleafs,
-                                          double2 *__restrict__ aslot, double (*tr)[16][17]) {
+                                          double2 *__restrict__ aslot, const unsigned *__restrict__ smask,
+                                          const int *__restrict__ cflags, double (*tr)[16][17]) {
   const int lane = threadIdx.x & 63;
   const int wv = wave_id();
   const int nb = gridDim.x, b = blockIdx.x;
@@ -896,6 +1089,11 @@ __device__ __forceinline__ void rbwd_body(const int2 *__restrict__ items, int ni
   double2 *as = aslot + inst.step_off * GV_SLOT;
   const bool transposed = ch.dir != 0;
   const int a16 = lane & 15, g4 = lane >> 4;
+  // mask words of a suffix chain the forward ran over the zero structure
+  // (k_rchains_gfwd_izs), for the recomputed steps
+  const unsigned *smc = smask && transposed && uload(cflags + inst.part_off + it.y)
+                            ? smask + (inst.site_off / GV_SLOT) * 2 + 1
+                            : nullptr;
 
   REnv eb;  // adjoint of the environment continuing the chain
 #pragma unroll
@@ -912,7 +1110,7 @@ __device__ __forceinline__ void rbwd_body(const int2 *__restrict__ items, int ni
       while (uload(P.ck_slot + ch.step0 + s0) < 0) --s0;
       renv_load_sym_nt(rslot(es, uload(P.ck_slot + ch.step0 + s0)), lane, E);
       for (int q = s0; q < st; ++q) {
-        radvance(P, sites, ch.step0 + q, lane, E);
+        radvance(P, sites, ch.step0 + q, lane, E, smc, tr[wv]);
         renv_symmetrize<false>(E, lane, tr[wv], nullptr, nullptr);  // as the forward pass did before step q + 1
       }
     }
@@ -1027,9 +1225,11 @@ extern "C" __global__ void __launch_bounds__(256) GV_FOUR_WAVES
 k_rchains_bwd(const int2 *__restrict__ items, int nitems, const InstDev *__restrict__ insts,
               const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
               const double *__restrict__ rgarenaT, const double2 *__restrict__ spills, double2 *__restrict__ adjspill,
-              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
+              const double2 *__restrict__ estore, const double2 *__restrict__ gleafs, double2 *__restrict__ aslot,
+              const unsigned *__restrict__ smask, const int *__restrict__ cflags) {
   __shared__ double tr[4][16][17];
-  rbwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, adjspill, estore, gleafs, aslot, tr);
+  rbwd_body<false>(items, nitems, insts, probs, rgarena, rgarenaT, spills, adjspill, estore, gleafs, aslot, smask,
+                   cflags, tr);
 }
 
 extern "C" __global__ void __launch_bounds__(256) GV_BWD_IZ_WAVES
@@ -1037,9 +1237,11 @@ k_rchains_bwd_iz(const int2 *__restrict__ items, int nitems, const InstDev *__re
                  const ProbDev *__restrict__ probs, const double *__restrict__ rgarena,
                  const double *__restrict__ rgarenaT, const double2 *__restrict__ spills,
                  double2 *__restrict__ adjspill, const double2 *__restrict__ estore,
-                 const double2 *__restrict__ gleafs, double2 *__restrict__ aslot) {
+                 const double2 *__restrict__ gleafs, double2 *__restrict__ aslot,
+                 const unsigned *__restrict__ smask, const int *__restrict__ cflags) {
   __shared__ double tr[4][16][17];
-  rbwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, adjspill, estore, gleafs, aslot, tr);
+  rbwd_body<true>(items, nitems, insts, probs, rgarena, rgarenaT, spills, adjspill, estore, gleafs, aslot, smask,
+                  cflags, tr);
 }
 
 // Per (instance, site): sum of its steps' real adjoints (radj_store's compact
@@ -1224,25 +1426,33 @@ hipError_t kl_site_masks(long long nslots, hipStream_t s, const double *rarena, 
 // sites without structure (rchains_merged_body).
 extern "C" __global__ void __launch_bounds__(256)
 k_chain_flags(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
-              const unsigned *__restrict__ smask, int *__restrict__ cflags) {
+              const unsigned *__restrict__ smask, int *__restrict__ cflags, int grad) {
   const InstDev inst = uload(insts + blockIdx.x);
   const ProbDev P = uload(probs + inst.prob);
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   if (c >= P.nchains) return;
-  cflags[inst.part_off + c] = chain_flag(P, c, smask + (inst.site_off / GV_SLOT) * 2);
+  cflags[inst.part_off + c] = chain_flag(P, c, smask + (inst.site_off / GV_SLOT) * 2, grad != 0);
 }
 
 hipError_t kl_chain_flags(int ninst, int max_chains, hipStream_t s, const InstDev *insts, const ProbDev *probs,
-                          const unsigned *smask, int *cflags) {
+                          const unsigned *smask, int *cflags, int grad) {
   if (ninst <= 0 || max_chains <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_chain_flags, dim3(ninst, (max_chains + 255) / 256), dim3(256), 0, s, insts, probs, smask,
-                     cflags);
+                     cflags, grad);
   return hipGetLastError();
 }
 
 hipError_t kl_rchains_gfwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                            const ProbDev *probs, const double *rgarena, long long nsite, double2 *spills, double *parts,
-                           double2 *estore, double2 *gleafs, double2 *leafs) {
+                           double2 *estore, double2 *gleafs, double2 *leafs, const double *farena,
+                           const unsigned *smask, const int *cflags) {
+  if (iz && smask) {
+    const long long nslots = nsite / GV_SLOT;
+    hipLaunchKernelGGL(k_rchains_gfwd_izs, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts,
+                       probs, rgarena, rgarena + nsite, farena, farena + nslots * GV_FSLOT, smask, cflags, spills,
+                       parts, estore, gleafs, leafs);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(iz ? k_rchains_gfwd_iz : k_rchains_gfwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
                      rgarena, rgarena + nsite, spills, parts, estore, gleafs, leafs);
   return hipGetLastError();
@@ -1250,9 +1460,10 @@ hipError_t kl_rchains_gfwd(int nitems, int wpb, bool iz, hipStream_t s, const in
 
 hipError_t kl_rchains_bwd(int nitems, int wpb, bool iz, hipStream_t s, const int2 *items, const InstDev *insts,
                           const ProbDev *probs, const double *rgarena, long long nsite, const double2 *spills,
-                          double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot) {
+                          double2 *adjspill, const double2 *estore, const double2 *gleafs, double2 *aslot,
+                          const unsigned *smask, const int *cflags) {
   hipLaunchKernelGGL(iz ? k_rchains_bwd_iz : k_rchains_bwd, dim3((nitems + wpb - 1) / wpb), dim3(64 * wpb), 0, s, items, nitems, insts, probs,
-                     rgarena, rgarena + nsite, spills, adjspill, estore, gleafs, aslot);
+                     rgarena, rgarena + nsite, spills, adjspill, estore, gleafs, aslot, smask, cflags);
   return hipGetLastError();
 }
 

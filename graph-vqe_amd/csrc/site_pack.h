@@ -20,6 +20,13 @@ namespace gv {
 // pair gives Y00, Y11, Y01 and Y10, each entry from the same products in the
 // same order as the separate products (the rest being exact zeros).
 #define GV_SM_DISJOINT 0x100u
+// Bit 9: the rows of M_0 are even and those of M_1 odd.  The bond-doubling
+// state of the reverse gradient has it in the transposed copy of every site
+// whose program ends with a CNOT's control half (the doubled bond's new bit
+// is its lowest, so A_i sits in the columns of parity i); the gradient's
+// forward chains then contract over the rows in the order 0, 2, .., 14,
+// 1, 3, .., 15, where the two components are a row split.
+#define GV_SM_PARITY 0x200u
 
 // Merged site fragment F = M_0 + M_1, row-major 16 x 16 per slot and copy.
 #define GV_FSLOT (GV_CHI * GV_CHI)
@@ -85,22 +92,35 @@ __device__ __forceinline__ void site_mask_slot(const double (&av)[2][4], double 
     return m;
   };
   // direct copy: rows of M_i are rows of A_i, columns are columns; transposed: swapped
+  auto parity = [](unsigned s0, unsigned s1) {
+    return ((s0 & 0xaaaau) == 0u && (s1 & 0x5555u) == 0u) ? GV_SM_PARITY : 0u;
+  };
   const unsigned direct = groups(rows[0]) | (groups(rows[1]) << 4) |
-                          ((cols[0] & cols[1]) == 0u ? GV_SM_DISJOINT : 0u) | (cols[1] << 16);
+                          ((cols[0] & cols[1]) == 0u ? GV_SM_DISJOINT : 0u) | parity(rows[0], rows[1]) |
+                          (cols[1] << 16);
   const unsigned transposed = groups(cols[0]) | (groups(cols[1]) << 4) |
-                              ((rows[0] & rows[1]) == 0u ? GV_SM_DISJOINT : 0u) | (rows[1] << 16);
+                              ((rows[0] & rows[1]) == 0u ? GV_SM_DISJOINT : 0u) | parity(cols[0], cols[1]) |
+                              (rows[1] << 16);
   if (lane == 0) {
     sm2[0] = direct;
     sm2[1] = transposed;
   }
 }
 
+// Structure the gradient's forward chains use: disjoint columns in the direct
+// copy (prefix chains, merged products), the parity split in the transposed
+// one (suffix chains, row-split products over the permuted rows).
+__device__ __forceinline__ bool site_structured_grad(unsigned mw, int dir) {
+  return (mw & (dir != 0 ? GV_SM_PARITY : GV_SM_DISJOINT)) != 0u;
+}
+
 // Chain flag of chain c of an instance: 1 when at most a quarter of its
 // steps are on sites without structure (the chain then runs over the merged
-// fragments).  sm: the instance's mask words (two per site: direct,
-// transposed).  Eight steps per round: their descriptor loads, then their
-// mask loads, are issued together instead of one dependent pair per step.
-__device__ __forceinline__ int chain_flag(const ProbDev &P, int c, const unsigned *sm) {
+// fragments; grad: the structure site_structured_grad asks for).  sm: the
+// instance's mask words (two per site: direct, transposed).  Eight steps per
+// round: their descriptor loads, then their mask loads, are issued together
+// instead of one dependent pair per step.
+__device__ __forceinline__ int chain_flag(const ProbDev &P, int c, const unsigned *sm, bool grad) {
   const ChainDev ch = P.chains[c];
   const unsigned *smc = sm + (ch.dir != 0 ? 1 : 0);
   int plain = 0;
@@ -110,9 +130,10 @@ __device__ __forceinline__ int chain_flag(const ProbDev &P, int c, const unsigne
     for (int q = 0; q < 8; ++q) site[q] = s0 + q < ch.nsteps ? P.steps[ch.step0 + s0 + q].site : -1;
     unsigned mw[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) mw[q] = site[q] >= 0 ? smc[2 * site[q]] : GV_SM_DISJOINT;
+    for (int q = 0; q < 8; ++q) mw[q] = site[q] >= 0 ? smc[2 * site[q]] : GV_SM_DISJOINT | GV_SM_PARITY;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) plain += site_structured(mw[q]) ? 0 : 1;
+    for (int q = 0; q < 8; ++q)
+      plain += (grad ? site_structured_grad(mw[q], ch.dir) : site_structured(mw[q])) ? 0 : 1;
   }
   return 4 * plain <= ch.nsteps ? 1 : 0;
 }

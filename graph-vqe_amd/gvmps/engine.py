@@ -165,7 +165,10 @@ class Batch:
 
     def set_sparsity(self, on=True):
         """Real energy forests skip exact-zero products of the sites' zero
-        structure (default on); energies are the same either way."""
+        structure (default on); energies are the same either way.  The real
+        gradient's forward chains use it too (suffix chains then contract
+        their rows in even-then-odd order), so gradients with it off agree
+        to rounding, not bit for bit."""
         check(lib().gvmps_batch_set_sparsity(self._h, int(bool(on))))
 
     def chain_flops(self):

@@ -171,8 +171,11 @@ int gvmps_batch_set_checkpoint(gvmps_batch *b, int stride);
  * chains read F (half the fragment bytes) and the rest keep the full sites.
  * The states of CNOT-entangled ansatze have this structure: in the 40-qubit
  * workload every bulk site's A_0 and A_1 occupy disjoint halves of the left
- * bond.  No reference counterpart (simulator_mps_dist.cu contracts dense
- * sites). */
+ * bond.  The real reverse gradient's forward chains (I/Z forests) use the
+ * structure of its bond-doubling state too, the suffix chains contracting
+ * over their rows in even-then-odd order: with it off, gradients agree to
+ * rounding rather than bit for bit.  No reference counterpart
+ * (simulator_mps_dist.cu contracts dense sites). */
 int gvmps_batch_set_sparsity(gvmps_batch *b, int on);
 
 /* FP64 flops the forest-chain launches of the batch's last energy call
