@@ -516,6 +516,9 @@ def bench_grads(args, prob, info, x, B):
     (e, g), dt_call, phase_ms, calls = timed_calls(lambda: batch.value_grad(x), args.grad_steps, batch)
     dt = dt_call * args.grad_steps
     rate = B / dt_call
+    # the products the last value_grad call's chain launches executed (the
+    # forward's zero-structure skips counted on the real path)
+    flops = batch.grad_flops()
     del batch
     torch.cuda.empty_cache()
     single = Batch([prob], [1])
@@ -527,10 +530,7 @@ def bench_grads(args, prob, info, x, B):
     names = ["bond_doubling_build", "forward_chains", "energy_reduce", "backward_chains", "site_reduce",
              "local_backward"]
     fwd_ms, bwd_ms = phase_ms[1] / c, phase_ms[3] / c
-    # complex count: 4 real products per complex product; the real path
-    # (real tape, no Y words) executes one
     real = bool(info['real_arith'])
-    flops = info['flops_grad_executed'] * B / (4.0 if real else 1.0)
     achieved = flops / ((fwd_ms + bwd_ms) * 1e-3) / 1e12
     return {
         "value": round(rate, 2), "unit": "adjoint grads/s", "steps": args.grad_steps,

@@ -452,6 +452,7 @@ void launch_energy_tail(Batch &B, hipStream_t stream, bool terms) {
       hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask, B.rfarena), "k_site_masks launch");
       hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags, 0),
                 "k_chain_flags launch");
+      B.mask_kind = 0;
     }
     for (int g = 0; g < B.max_gen; ++g) {
       const int ni = B.n_items[g];
@@ -625,6 +626,7 @@ void launch_value_grad(Batch &B, const double *params, hipStream_t stream) {
       hip_check(kl_site_masks(B.n_site_total / GV_SLOT, stream, B.rarena, B.smask, B.rfarena), "k_site_masks launch");
       hip_check(kl_chain_flags(ninst, B.max_chains, stream, B.d_inst, B.d_probs, B.smask, B.cflags, 1),
                 "k_chain_flags launch");
+      B.mask_kind = 1;
     }
     B.mark(1, stream);
     // the energy launches, deferred term ends by the leaf kernel
@@ -997,6 +999,7 @@ int gvmps_batch_chain_flops(gvmps_batch *b, double *flops) {
     std::vector<int> flag;  // I/Z forests: chains on the merged fragments (the rest run full products)
     const bool real = B.real && B.arith == 0;
     if (real && B.use_smask) {
+      if (B.mask_kind != 0) throw std::runtime_error("chain flops: the batch's last call was not an energy call");
       mask.resize(2 * (size_t)(B.n_site_total / GV_SLOT));
       hip_check(hipMemcpy(mask.data(), B.smask, mask.size() * sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
       if (B.iz) {
@@ -1034,6 +1037,54 @@ int gvmps_batch_chain_flops(gvmps_batch *b, double *flops) {
             else products = both ? 6.0 : 4.0;
           }
           total += products * prod;
+        }
+      }
+    }
+    *flops = total;
+  })
+}
+
+int gvmps_batch_grad_flops(gvmps_batch *b, double *flops) {
+  GV_TRY({
+    Batch &B = *reinterpret_cast<Batch *>(b);
+    if (!flops) throw std::runtime_error("null output");
+    hip_check(hipSetDevice(B.device), "hipSetDevice");
+    const bool real = B.real && B.arith == 0;
+    double total = 0.0;
+    if (!real) {  // complex kernels: the planner's count (4 real products per complex one)
+      for (const InstDev &in : B.inst) total += B.probs[in.prob]->flops_grad;
+      *flops = total;
+      return 0;
+    }
+    std::vector<unsigned> mask;
+    std::vector<int> flag;  // chains the forward ran over the zero structure
+    if (B.iz && B.use_smask && B.mask_kind == 1) {
+      mask.resize(2 * (size_t)(B.n_site_total / GV_SLOT));
+      hip_check(hipMemcpy(mask.data(), B.smask, mask.size() * sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
+      long long nparts = 0;
+      for (const InstDev &in : B.inst)
+        nparts = std::max(nparts, in.part_off + (long long)B.probs[in.prob]->chains.size());
+      flag.resize(nparts);
+      hip_check(hipMemcpy(flag.data(), B.cflags, flag.size() * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    const double prod = 2.0 * GV_CHI * GV_CHI * GV_CHI;  // one real 16^3 product
+    for (const InstDev &in : B.inst) {
+      const Problem &P = *B.probs[in.prob];
+      const long long slot0 = in.site_off / GV_SLOT;
+      for (size_t c = 0; c < P.chains.size(); ++c) {
+        const ChainDev &ch = P.chains[c];
+        const bool structured = !flag.empty() && flag[in.part_off + c] != 0;
+        for (int st = 0; st < ch.nsteps; ++st) {
+          const StepDev &sd = P.steps[ch.step0 + st];
+          const bool iz = sd.need & 1, xy = sd.need & 2;
+          // forward: X_0, X_1 and a Y pair per pass; backward: G, Ebar, Mbar pairs
+          double fwd = 2.0 + (iz ? 2.0 : 0.0) + (xy ? 2.0 : 0.0);
+          const double bwd = (iz && xy) ? 8.0 : 6.0;
+          if (structured) {
+            const unsigned mw = mask[2 * (slot0 + sd.site) + (ch.dir != 0 ? 1 : 0)];
+            if (mw & (ch.dir != 0 ? 0x200u : 0x100u)) fwd = 2.0;  // row-split over permuted rows / merged pair
+          }
+          total += (fwd + bwd) * prod;
         }
       }
     }

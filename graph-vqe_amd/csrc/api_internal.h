@@ -70,6 +70,7 @@ struct Batch {
   double *rfarena = nullptr;  // merged fragments A_0 + A_1 of each slot, direct then transposed (k_site_masks)
   int *cflags = nullptr;      // per (instance, chain): its steps run on the merged fragments (k_chain_flags)
   bool use_smask = true;      // chain steps skip exact-zero products (gvmps_batch_set_sparsity)
+  int mask_kind = -1;         // smask / cflags last written by: 0 an energy call, 1 a value_grad call
   const int2 *gen_items(int g) const { return order ? items_len[g] : items[g]; }
   const int2 *ggen_items(int g) const { return order ? gitems_len[g] : gitems[g]; }
   const int2 *bgen_items(int g) const { return order ? bitems_len[g] : bitems[g]; }

@@ -65,6 +65,7 @@ _PROTOS = {
     'gvmps_batch_set_checkpoint': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_set_sparsity': [ctypes.c_void_p, ctypes.c_int],
     'gvmps_batch_chain_flops': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
+    'gvmps_batch_grad_flops': [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
     'gvmps_batch_evaluate': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     'gvmps_batch_capture': [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

@@ -178,6 +178,13 @@ class Batch:
         check(lib().gvmps_batch_chain_flops(self._h, ctypes.byref(out)))
         return out.value
 
+    def grad_flops(self):
+        """FP64 flops the chain launches of the last value_grad call executed
+        (forward and backward; the forward's zero-structure skips counted)."""
+        out = ctypes.c_double()
+        check(lib().gvmps_batch_grad_flops(self._h, ctypes.byref(out)))
+        return out.value
+
     def set_checkpoint(self, stride):
         """value_grad keeps every stride-th step environment of a chain and
         recomputes the rest in the backward pass (same results)."""

@@ -186,6 +186,14 @@ int gvmps_batch_set_sparsity(gvmps_batch *b, int on);
  * marks back with a blocking copy.  No reference counterpart. */
 int gvmps_batch_chain_flops(gvmps_batch *b, double *flops);
 
+/* FP64 flops the forward and backward chain launches of the batch's last
+ * value_grad call executed, summed over its instances: per chain step the
+ * forward's products (merged or row-split on the steps that ran over the
+ * zero structure) and the backward's.  The complex path reports the
+ * planner's count.  For the adjoint's MFMA fraction; reads the marks back
+ * with a blocking copy.  No reference counterpart. */
+int gvmps_batch_grad_flops(gvmps_batch *b, double *flops);
+
 /* Launch configuration of the forest-chain kernels: wavefronts per workgroup
  * (1, 2 or 4) and work-item order (0 instance-major, 1 (default) longest
  * chains first within each XCD's share of the instance-major items, so each

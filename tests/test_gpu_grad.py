@@ -297,6 +297,27 @@ def test_checkpoint_strides_are_bitwise_identical(cuda):
             assert np.array_equal(g.cpu().numpy(), g1), (arith, stride)
 
 
+def test_grad_flops_count_the_forward_skips(cuda):
+    """The executed-flop count of value_grad: with the zero structure the
+    40-qubit forward issues half its products on the bulk steps (2 of the
+    10 real products per step), so the count lies between 70% and 90% of
+    the full-product count; with skipping off it is the full count, and
+    the gradients agree with the skipping run to rounding."""
+    w = workloads.config(1)
+    p = Problem(w['n'], w['ops'], w['nparams'], w['words'], w['coeffs'], **SVD)
+    X = w['x'][:4]
+    b = Batch([p], [4])
+    print('case: spsa-40q x 4, gradient flop count with and without skips', flush=True)
+    e, g = b.value_grad(X)
+    full = p.info()['flops_grad_executed'] * 4 / 4.0  # one real product per complex one
+    assert 0.7 * full < b.grad_flops() < 0.9 * full
+    b.set_sparsity(False)
+    e2, g2 = b.value_grad(X)
+    assert b.grad_flops() == pytest.approx(full, rel=1e-12)
+    np.testing.assert_allclose(e2.cpu().numpy(), e.cpu().numpy(), rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g2.cpu().numpy(), g.cpu().numpy(), rtol=0, atol=1e-9)
+
+
 def test_gradient_autotuner_configs_are_bitwise_identical(cuda):
     """Every launch shape and checkpoint stride the gradient autotuner tries
     gives bitwise the same energies and gradients, and the tuner leaves the
