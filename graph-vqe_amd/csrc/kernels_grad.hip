@@ -72,7 +72,7 @@ extern "C" __global__ void __launch_bounds__(64)
 k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
               const double *__restrict__ params, double2 *__restrict__ garena, double2 *__restrict__ rotin,
               int *__restrict__ status) {
-  __shared__ double2 buf[2][GV_SLOT];
+  __shared__ double2 buf[GV_SLOT];  // one wave: read into registers, barrier, write back
   const int lane = threadIdx.x;
   const InstDev inst = insts[blockIdx.y];
   const int s = blockIdx.x;
@@ -83,10 +83,9 @@ k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int e = lane + 64 * k;
-    buf[0][e] = make_double2(e == 0 ? 1.0 : 0.0, 0.0);  // |0> of a (1,2,1) site
+    buf[e] = make_double2(e == 0 ? 1.0 : 0.0, 0.0);  // |0> of a (1,2,1) site
   }
   __syncthreads();
-  int cur = 0;
   const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1];
   for (int o = o0; o < o1; ++o) {
     const LocalOp op = P.lops[o];
@@ -94,8 +93,8 @@ k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
     local_matrix(op, x, P.fixes, U, dU);
     // rotation inputs are kept only for the reverse pass (rotin == nullptr: energy-path preparation)
     double2 *keep = (op.kind == LO_ROT && rotin != nullptr) ? rotin + (inst.rot_off + op.rot) * GV_SLOT : nullptr;
-    const double2 *in = buf[cur];
-    double2 *out = buf[cur ^ 1];
+    const double2 *in = buf;
+    double2 w[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int e = lane + 64 * k;
@@ -118,14 +117,16 @@ k_build_local(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
         default:  // 2x2 gate on the physical index
           v = cadd(cmul(i ? U[2] : U[0], in[sidx(l, 0, r)]), cmul(i ? U[3] : U[1], in[sidx(l, 1, r)]));
       }
-      out[e] = v;
+      w[k] = v;
     }
     __syncthreads();
-    cur ^= 1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) buf[lane + 64 * k] = w[k];
+    __syncthreads();
   }
   double2 *dst = garena + inst.site_off + (size_t)s * GV_SLOT;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) dst[lane + 64 * k] = buf[cur][lane + 64 * k];
+  for (int k = 0; k < 8; ++k) dst[lane + 64 * k] = buf[lane + 64 * k];
 }
 
 // --------------------------------------------------------------- forward
@@ -554,7 +555,7 @@ k_site_reduce(const InstDev *__restrict__ insts, const ProbDev *__restrict__ pro
 extern "C" __global__ void __launch_bounds__(64)
 k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs, const double *__restrict__ params,
            const double2 *__restrict__ abar, const double2 *__restrict__ rotin, double *__restrict__ rgrad) {
-  __shared__ double2 buf[2][GV_SLOT];
+  __shared__ double2 buf[GV_SLOT];  // one wave: read into registers, barrier, write back
   const int lane = threadIdx.x;
   const InstDev inst = insts[blockIdx.y];
   const int s = blockIdx.x;
@@ -563,16 +564,15 @@ k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
   const double *x = params + inst.param_off;
   const double2 *src = abar + inst.site_off + (size_t)s * GV_SLOT;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) buf[0][lane + 64 * k] = src[lane + 64 * k];
+  for (int k = 0; k < 8; ++k) buf[lane + 64 * k] = src[lane + 64 * k];
   __syncthreads();
-  int cur = 0;
   const int o0 = P.lop_begin[s], o1 = P.lop_begin[s + 1] - (P.site_tail[s] & 3);
   for (int o = o1 - 1; o >= o0; --o) {
     const LocalOp op = P.lops[o];
     double2 U[4], dU[4];
     local_matrix(op, x, P.fixes, U, dU);
-    const double2 *in = buf[cur];
-    double2 *out = buf[cur ^ 1];
+    const double2 *in = buf;
+    double2 w[8];
     if (op.kind == LO_ROT) {
       const double2 *T = rotin + (inst.rot_off + op.rot) * GV_SLOT;
       double part = 0.0;
@@ -608,10 +608,12 @@ k_site_bwd(const InstDev *__restrict__ insts, const ProbDev *__restrict__ probs,
         default:  // Tbar_k[l,j,r] = sum_i conj(U[i][j]) Tbar[l,i,r]
           v = cadd(cmul(cconj(i ? U[1] : U[0]), in[sidx(l, 0, r)]), cmul(cconj(i ? U[3] : U[2]), in[sidx(l, 1, r)]));
       }
-      out[e] = v;
+      w[k] = v;
     }
     __syncthreads();
-    cur ^= 1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) buf[lane + 64 * k] = w[k];
+    __syncthreads();
   }
 }
 
